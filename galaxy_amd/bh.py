@@ -1,0 +1,258 @@
+"""Python mirror of the GALAXY BH #2D backend API over the gfx950 C ABI.
+
+Class and method names follow runtime/src/nbody_gpu.rs (NbodyGpu) and
+runtime/src/nbody_parallel.rs (ParallelTreeRuntime); errors raise BhError
+with the library's message (the reference's Box<dyn Error> strings).
+Record arrays use the frozen transfer ABI (runtime/src/nbody_gpu.rs:22-83):
+
+  state (n, 8) float32 = GpuEvolveState {x, y, m, 0, vx, vy, 0, 0}
+  accel (n, 4) float32 = GpuAccel {ax, ay, 0, 0}
+  entry (n, 4) uint32  = GpuEntry {code, body, 0, 0}
+  cell  (k, 16) uint32 = GpuCell with float fields as bit patterns
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+G_KPC3_PER_MSUN_MYR2 = 4.498502151575286e-12  # nbody/src/lib.rs:6
+MAX_PARALLEL_TREE_BODIES = 65536              # nbody_parallel.rs:20
+
+
+class BhError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise BhError(L.lib().gbh_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Config:
+    """galaxy_nbody::Config (nbody/src/lib.rs:27-46)."""
+
+    theta: float = 0.5
+    softening_kpc: float = 0.05
+    g: float = G_KPC3_PER_MSUN_MYR2
+    bucket: int = 4
+    max_depth: int = 48
+
+    def c(self):
+        return L.GbhConfig(self.theta, self.softening_kpc, self.g, self.bucket, self.max_depth)
+
+
+def as_state(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 8:
+        raise ValueError("state records must have shape (n, 8)")
+    return a
+
+
+class NbodyGpu:
+    """Device handle (NbodyGpu::new, nbody_gpu.rs:331-621)."""
+
+    def __init__(self, adapter: str | None = None, allow_software: bool = False):
+        h = C.c_void_p()
+        _check(L.lib().gbh_device_open(adapter.encode() if adapter else None, int(allow_software),
+                                       C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gbh_device_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self) -> dict:
+        n = C.c_size_t()
+        buf = C.create_string_buffer(2048)
+        _check(L.lib().gbh_device_info_json(self._h, buf, 2048, C.byref(n)))
+        return json.loads(buf.value.decode())
+
+    # -- evolving state (nbody_gpu.rs:786-1118) --
+    def create_evolving_state(self, states) -> "EvolvingState":
+        return EvolvingState(self, as_state(states))
+
+    def kick_drift(self, evolving: "EvolvingState", dt: float):
+        t = L.StageTiming()
+        _check(L.lib().gbh_kick_drift(evolving._h, float(dt), C.byref(t)))
+        return t.as_dict()
+
+    def final_kick(self, evolving: "EvolvingState", dt: float):
+        t = L.StageTiming()
+        _check(L.lib().gbh_final_kick(evolving._h, float(dt), C.byref(t)))
+        return t.as_dict()
+
+    def read_evolving_state(self, evolving: "EvolvingState") -> np.ndarray:
+        out = np.zeros((evolving.count, 8), np.float32)
+        _check(L.lib().gbh_state_read(evolving._h, _p(out), None))
+        return out
+
+    def read_evolving_accelerations(self, evolving: "EvolvingState") -> np.ndarray:
+        out = np.zeros((evolving.count, 4), np.float32)
+        _check(L.lib().gbh_accel_read(evolving._h, _p(out), None))
+        return out
+
+    # -- BH #2B1 / #2B2 host-tree ingest (nbody_gpu.rs:640-938) --
+    def traverse(self, settings, bodies, entries, cells) -> np.ndarray:
+        settings = np.ascontiguousarray(settings, np.uint32)
+        bodies = np.ascontiguousarray(bodies, np.uint32)
+        entries = np.ascontiguousarray(entries, np.uint32)
+        cells = np.ascontiguousarray(cells, np.uint32)
+        out = np.zeros((int(settings[0]), 4), np.float32)
+        _check(L.lib().gbh_traverse_packed(self._h, _p(settings), _p(bodies), _p(entries), _p(cells),
+                                           _p(out), None))
+        return out
+
+    def force_into(self, evolving: "EvolvingState", settings, bodies, entries, cells):
+        settings = np.ascontiguousarray(settings, np.uint32)
+        bodies = np.ascontiguousarray(bodies, np.uint32)
+        entries = np.ascontiguousarray(entries, np.uint32)
+        cells = np.ascontiguousarray(cells, np.uint32)
+        _check(L.lib().gbh_force_packed(evolving._h, _p(settings), _p(bodies), _p(entries), _p(cells),
+                                        None))
+
+
+class EvolvingState:
+    """EvolvingState (nbody_gpu.rs:132-138)."""
+
+    def __init__(self, gpu: NbodyGpu, states: np.ndarray):
+        self.gpu = gpu
+        self.count = len(states)
+        h = C.c_void_p()
+        _check(L.lib().gbh_state_create(gpu._h, _p(states), self.count, C.byref(h), None))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gbh_state_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def buffer_bytes(self) -> int:
+        return int(L.lib().gbh_state_bytes(self._h))
+
+
+@dataclass
+class TreeEvidence:
+    """ParallelGpuTreeEvidence (nbody_parallel.rs:83-91)."""
+
+    meta: np.ndarray       # (8,) uint32: data[4] + bounds bits[4]
+    entries: np.ndarray    # (n, 4) uint32
+    cells: np.ndarray      # (active, 16) uint32
+    active_cell_count: int
+    leaf_count: int
+    max_depth: int
+
+    @property
+    def bounds(self) -> np.ndarray:
+        return self.meta[4:8].view(np.float32)
+
+
+class ParallelGpuTree:
+    def __init__(self, runtime: "ParallelTreeRuntime", evolving: EvolvingState, large: bool = False):
+        self.runtime = runtime
+        self.evolving = evolving
+        self.count = evolving.count
+        h = C.c_void_p()
+        fn = L.lib().gbh_tree_create_large if large else L.lib().gbh_tree_create
+        _check(fn(runtime._h, evolving._h, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gbh_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class ParallelTreeRuntime:
+    """BH #2D parallel tree runtime (nbody_parallel.rs:93-993)."""
+
+    def __init__(self, gpu: NbodyGpu):
+        self.gpu = gpu
+        h = C.c_void_p()
+        _check(L.lib().gbh_runtime_create(gpu._h, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gbh_runtime_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def create_tree(self, gpu: NbodyGpu, evolving: EvolvingState, large: bool = False) -> ParallelGpuTree:
+        return ParallelGpuTree(self, evolving, large)
+
+    def rebuild(self, gpu, evolving: EvolvingState, tree: ParallelGpuTree, config: Config) -> dict:
+        t = L.BuildTiming()
+        _check(L.lib().gbh_tree_rebuild(self._h, evolving._h, tree._h, C.byref(config.c()), C.byref(t)))
+        return t.as_dict()
+
+    def force(self, gpu, evolving: EvolvingState, tree: ParallelGpuTree, config: Config) -> dict:
+        t = L.StageTiming()
+        _check(L.lib().gbh_force(self._h, evolving._h, tree._h, C.byref(config.c()), C.byref(t)))
+        return t.as_dict()
+
+    def force_evidence(self, evolving: EvolvingState, tree: ParallelGpuTree, config: Config):
+        """Per-target (visited, accepted, direct) counts and interaction-list hashes."""
+        stats = np.zeros((evolving.count, 3), np.uint32)
+        hsh = np.zeros(evolving.count, np.uint64)
+        _check(L.lib().gbh_force_evidence(self._h, evolving._h, tree._h, C.byref(config.c()),
+                                          _p(stats), _p(hsh)))
+        return stats, hsh
+
+    def counts(self, tree: ParallelGpuTree):
+        a, lf, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(L.lib().gbh_tree_counts(tree._h, C.byref(a), C.byref(lf), C.byref(md)))
+        return int(a.value), int(lf.value), int(md.value)
+
+    def read_evidence(self, gpu, tree: ParallelGpuTree) -> TreeEvidence:
+        active, _, _ = self.counts(tree)
+        meta = np.zeros(8, np.uint32)
+        entries = np.zeros((tree.count, 4), np.uint32)
+        cells = np.zeros((active, 16), np.uint32)
+        a, lf, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(L.lib().gbh_tree_read_evidence(tree._h, _p(meta), _p(entries), _p(cells), active,
+                                              C.byref(a), C.byref(lf), C.byref(md)))
+        return TreeEvidence(meta, entries, cells, int(a.value), int(lf.value), int(md.value))
+
+    def read_flat(self, tree: ParallelGpuTree):
+        """Tree in BH #2A preorder packed like pack_flat_tree -> (entries, cells, position_by_body)."""
+        active, _, _ = self.counts(tree)
+        entries = np.zeros((tree.count, 4), np.uint32)
+        cells = np.zeros((active, 16), np.uint32)
+        pos = np.zeros(tree.count, np.uint32)
+        k = C.c_uint32()
+        _check(L.lib().gbh_tree_read_flat(tree._h, _p(entries), _p(cells), active, C.byref(k), _p(pos)))
+        return entries, cells[: k.value], pos
+
+    def last_force_ms(self, tree: ParallelGpuTree) -> float:
+        """GPU time of the last force traversal kernel (HIP events)."""
+        return float(L.lib().gbh_tree_last_force_ms(tree._h))
+
+    def buffer_bytes(self, tree: ParallelGpuTree) -> int:
+        return int(L.lib().gbh_tree_buffer_bytes(tree._h))
+
+
+def sparse_layout_bytes(n: int) -> int:
+    return int(L.lib().gbh_sparse_layout_bytes(n))

@@ -1,0 +1,1098 @@
+// gfx950 HIP kernels of the resident Barnes-Hut step (tree build, traversal, KDK).
+//
+// One code path serves one simulation (the drop-in) and many (batched/served):
+// every kernel walks a table of "tiles", each tile belongs to exactly one
+// simulation, and every simulation owns its own device buffers described by a
+// SimDesc.  The build reproduces the BH #2A flat tree (nbody/src/flat.rs)
+// bit-for-bit: root bounds, Morton keys, cell geometry and aggregates are
+// computed in f64 in the reference's operation order and rounded to f32 once,
+// as pack_flat_tree does (runtime/src/nbody_gpu.rs:190-229).  Cells are laid
+// out in flat.rs preorder, which lets the traversal run stackless.
+//
+// Arithmetic discipline: built with -ffp-contract=off and correctly rounded
+// f32 division / square root, so every f32 and f64 operation below is one
+// IEEE round-to-nearest operation in source order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gbh {
+
+constexpr uint32_t kNoChild = 0xffffffffu;
+constexpr uint32_t kTile = 256;          // bodies per tile: bounds, KDK
+constexpr uint32_t kTopoTile = 1024;     // sorted positions per topology workgroup
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kRadix = 256;
+constexpr uint32_t kWaveTargets = 64;    // traversal targets per wavefront
+constexpr uint32_t kLevels = 17;         // Morton depths 0..16
+constexpr uint32_t kInfoLeaf = 1u << 8;
+constexpr uint32_t kInfoSpanShift = 9;   // (span slot + 1) << 9 for internal cells that cross tiles
+constexpr uint32_t kSpanSlotsLds = 4096; // spanning cells aggregated in LDS
+
+// Control words shared by the kernels of one simulation set.
+enum : uint32_t {
+    kCtlSortTicket = 0,  // 4 words, one per radix pass
+    kCtlTopoTicket = 4,
+    kCtlQueue = 5,
+    kCtlError = 6,
+    kCtlWords = 16,
+};
+
+// Error bits (sticky in kCtlError and SimMeta::err).
+enum : uint32_t {
+    kErrLookback = 1u,
+    kErrCellCapacity = 2u,
+    kErrSpanCapacity = 4u,
+};
+
+// Traversal record, 32 bytes, in flat.rs preorder.  Non-leaf cells descend to
+// index + 1; every cell skips to `skip` (the next cell after its subtree).
+struct __attribute__((aligned(32))) CellT {
+    float comx, comy, mass, T;  // T: smallest d2 that passes the opening test
+    uint32_t start, end, skip, info;
+};
+
+struct Agg {  // f64 aggregates of a cell (mass, centre of mass)
+    double m, x, y, pad;
+};
+
+struct SimMeta {
+    double root[4];         // f64 root (cx, cy, half) of root_bounds
+    float bounds_f32[4];    // TreeMeta.bounds
+    float T[20];            // per-depth acceptance thresholds (17 used)
+    uint32_t ncells;
+    uint32_t nspan;
+    uint32_t err;
+    uint32_t theta_bits;    // f32 theta the thresholds were computed for
+    uint32_t hist[4 * 256]; // per-pass digit histograms of the Morton keys
+};
+
+struct __attribute__((aligned(16))) SimDesc {
+    uint32_t n, cell_cap, bucket, maxd;
+    float theta, eps, G, dt;
+    uint32_t first_tile, first_sort_tile, first_topo_tile, span_cap;
+    // resident state (SoA)
+    float2* pos;
+    float* mass;
+    float2* vel;
+    float2* acc;
+    // radix sort ping-pong (local body index values)
+    uint32_t* key0;
+    uint32_t* key1;
+    uint32_t* val0;
+    uint32_t* val1;
+    // sorted bodies
+    float4* sbody;     // (x, y, m, 0) in sorted order
+    uint32_t* pos_of;  // body -> sorted position
+    // topology per sorted position
+    uint32_t* cbase;   // preorder index of the column's first cell
+    uint32_t* cparent; // start of the parent group of the column head
+    uint8_t* cdmin;    // depth of the column head (17: no cell starts here)
+    uint8_t* ccount;   // cells in the column
+    // cells (preorder)
+    CellT* cells;
+    float4* cgeom;     // f32 (cx, cy, half, 0)
+    uint4* cchild;     // children (kNoChild when absent)
+    Agg* cagg;         // f64 aggregates
+    uint32_t* span;    // spanning internal cells (span_cap entries)
+    float4* bpart;     // bounds partials, one per kTile bodies
+    SimMeta* meta;
+};
+
+struct TileRef {
+    uint32_t sim;
+    uint32_t base;
+};
+
+// ----------------------------------------------------------------------------
+// wavefront primitives (wave64, DPP within 16-lane rows + lane reads across rows)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t dpp_shr_u32(uint32_t v, uint32_t identity, int ctrl_sel) {
+    switch (ctrl_sel) {
+    case 1: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x111, 0xf, 0xf, false);
+    case 2: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x112, 0xf, 0xf, false);
+    case 4: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x114, 0xf, 0xf, false);
+    default: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x118, 0xf, 0xf, false);
+    }
+}
+
+// Minimum over all 64 lanes (every lane must be active).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, dpp_shr_u32(v, 0xffffffffu, 1));
+    v = min(v, dpp_shr_u32(v, 0xffffffffu, 2));
+    v = min(v, dpp_shr_u32(v, 0xffffffffu, 4));
+    v = min(v, dpp_shr_u32(v, 0xffffffffu, 8));
+    uint32_t a = __builtin_amdgcn_readlane(v, 15);
+    uint32_t b = __builtin_amdgcn_readlane(v, 31);
+    uint32_t c = __builtin_amdgcn_readlane(v, 47);
+    uint32_t d = __builtin_amdgcn_readlane(v, 63);
+    return min(min(a, b), min(c, d));
+}
+
+// Inclusive prefix sum over the 64 lanes (every lane must be active).
+__device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
+    v += dpp_shr_u32(v, 0u, 1);
+    v += dpp_shr_u32(v, 0u, 2);
+    v += dpp_shr_u32(v, 0u, 4);
+    v += dpp_shr_u32(v, 0u, 8);
+    uint32_t r0 = __builtin_amdgcn_readlane(v, 15);
+    uint32_t r1 = __builtin_amdgcn_readlane(v, 31);
+    uint32_t r2 = __builtin_amdgcn_readlane(v, 47);
+    uint32_t row = lane_id() >> 4;
+    uint32_t add = (row > 0 ? r0 : 0u) + (row > 1 ? r1 : 0u) + (row > 2 ? r2 : 0u);
+    return v + add;
+}
+
+// Block-wide exclusive prefix sum; returns the block total.  `scratch` holds one
+// word per wavefront.  Every thread of the block must call it.
+template <uint32_t kThreads>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* scratch, uint32_t& total) {
+    constexpr uint32_t kWaves = kThreads / 64;
+    uint32_t incl = wave_incl_sum_u32(v);
+    uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 63) scratch[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kWaves; ++i) {
+        uint32_t s = scratch[i];
+        before += i < w ? s : 0u;
+        all += s;
+    }
+    __syncthreads();
+    total = all;
+    return before + incl - v;
+}
+
+// ----------------------------------------------------------------------------
+// decoupled look-back flags: status in the top two bits, value in the low 30
+// ----------------------------------------------------------------------------
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kFlagValue = (1u << 30) - 1;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ void flag_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t flag_load(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of tile `t` over earlier tiles of the same segment, whose
+// flags are at flags[k * stride].  The segment's first tile publishes an
+// inclusive value without looking back, so every wait ends.
+__device__ uint32_t lookback(uint32_t* flags, uint32_t stride, uint32_t t, uint32_t* err) {
+    uint32_t excl = 0;
+    uint32_t k = t;
+    while (k > 0) {
+        --k;
+        uint32_t f = 0;
+        for (uint32_t spins = 0;; ++spins) {
+            f = flag_load(flags + (size_t)k * stride);
+            if (f != 0) break;
+            if (spins >= kSpinLimit) {
+                atomicOr(err, kErrLookback);
+                f = kFlagInc;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        excl += f & kFlagValue;
+        if (f & kFlagInc) break;
+    }
+    return excl;
+}
+
+// ----------------------------------------------------------------------------
+// Morton keys (flat.rs:67-94) and geometry (flat.rs:96-103)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0x0000ffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quantize_axis(double value, double center, double half) {
+    double low = center - half;
+    double width = half * 2.0;
+    double scaled = floor((value - low) / width * 65536.0);
+    if (!(scaled >= 0.0)) return 0u;  // clamp: NaN and negatives -> 0 (Rust `as u16`)
+    if (scaled > 65535.0) return 65535u;
+    return (uint32_t)scaled;
+}
+
+// Number of leading 2-bit Morton digits shared by two keys (16 when equal).
+__device__ __forceinline__ uint32_t lcp2(uint32_t a, uint32_t b) {
+    uint32_t x = a ^ b;
+    return x == 0 ? 16u : (uint32_t)__clz((int)x) >> 1;
+}
+
+// prefix at depth d: the top 2d bits
+__device__ __forceinline__ bool same_prefix(uint32_t a, uint32_t b, uint32_t d) {
+    return d == 0 || ((a ^ b) >> (32u - 2u * d)) == 0u;
+}
+
+// f32 opening threshold: the smallest positive d2 with fl(w / fl(sqrt(d2))) < theta
+// (nbody.wgsl:105-108 with w = 2*half).  NaN when no d2 passes (theta <= 0).
+__device__ float accept_threshold(float half, float theta) {
+    const float w = 2.0f * half;
+    auto pass = [&](uint32_t bits) {
+        float d2 = __uint_as_float(bits);
+        float s = __builtin_sqrtf(d2);
+        return (w / s) < theta;
+    };
+    const uint32_t inf_bits = 0x7f800000u;
+    if (!pass(inf_bits)) return __uint_as_float(0x7fc00000u);
+    if (pass(1u)) return __uint_as_float(1u);
+    uint32_t lo = 1u, hi = inf_bits;  // pass(lo) false, pass(hi) true
+    while (hi - lo > 1u) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (pass(mid)) hi = mid; else lo = mid;
+    }
+    return __uint_as_float(hi);
+}
+
+// point_mass (nbody.wgsl:35-41) with inverseSqrt = 1 / sqrt, each correctly rounded.
+__device__ __forceinline__ void point_mass(float tx, float ty, float sx, float sy, float m,
+                                           float eps2, float G, float& ax, float& ay) {
+    float dx = sx - tx;
+    float dy = sy - ty;
+    float dd = dx * dx;
+    float ee = dy * dy;
+    float r2 = (dd + ee) + eps2;
+    float s = __builtin_sqrtf(r2);
+    float inv_r = 1.0f / s;
+    float scale = G * m;
+    scale = scale * inv_r;
+    scale = scale * inv_r;
+    scale = scale * inv_r;
+    ax = ax + dx * scale;
+    ay = ay + dy * scale;
+}
+
+__device__ __forceinline__ void mix64(unsigned long long& h, unsigned long long w) {
+    h ^= w;
+    h *= 0x00000100000001b3ull;
+    h ^= (w << 23) | (w >> 41);
+}
+
+// ----------------------------------------------------------------------------
+// K1: kick/drift (+ optional preceding final kick) fused with the bounds partials
+// mode 0: bounds only; 1: kick/drift then bounds; 2: final kick + kick/drift then bounds
+// (nbody_evolve.wgsl:19-46, nbody_tree_parallel_bounds.wgsl:51-76)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ sims,
+                                                  const TileRef* __restrict__ tiles,
+                                                  uint32_t* __restrict__ ctl,
+                                                  uint32_t* __restrict__ tflags,
+                                                  uint32_t ntopo, int mode) {
+    __shared__ float s_red[4][kTile / 64];
+    const TileRef tr = tiles[blockIdx.x];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t i = tr.base + threadIdx.x;
+    const bool valid = i < S.n;
+    if (blockIdx.x == 0 && threadIdx.x < 6) ctl[threadIdx.x] = 0u;  // tickets + queue
+    // one look-back flag of the topology scan per block (there are fewer
+    // topology tiles than bounds tiles)
+    if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0u;
+    if (tr.base == 0) {
+        for (uint32_t k = threadIdx.x; k < 4 * 256; k += kTile) S.meta->hist[k] = 0u;
+    }
+    float x = 0.f, y = 0.f;
+    if (valid) {
+        float2 p = S.pos[i];
+        if (mode != 0) {
+            const float dt = S.dt;
+            const float h = 0.5f * dt;
+            float2 v = S.vel[i];
+            float2 a = S.acc[i];
+            if (mode == 2) {  // bh_final_kick of the previous step
+                v.x = v.x + h * a.x;
+                v.y = v.y + h * a.y;
+            }
+            float hvx = v.x + h * a.x;  // bh_kick_drift
+            float hvy = v.y + h * a.y;
+            p.x = p.x + dt * hvx;
+            p.y = p.y + dt * hvy;
+            S.pos[i] = p;
+            S.vel[i] = make_float2(hvx, hvy);
+        }
+        x = p.x;
+        y = p.y;
+    }
+    float mnx = valid ? x : __builtin_inff(), mny = valid ? y : __builtin_inff();
+    float mxx = valid ? x : -__builtin_inff(), mxy = valid ? y : -__builtin_inff();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, off));
+        mny = fminf(mny, __shfl_xor(mny, off));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_red[0][w] = mnx;
+        s_red[1][w] = mny;
+        s_red[2][w] = mxx;
+        s_red[3][w] = mxy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < kTile / 64; ++k) {
+            mnx = fminf(mnx, s_red[0][k]);
+            mny = fminf(mny, s_red[1][k]);
+            mxx = fmaxf(mxx, s_red[2][k]);
+            mxy = fmaxf(mxy, s_red[3][k]);
+        }
+        S.bpart[tr.base / kTile] = make_float4(mnx, mny, mxx, mxy);
+    }
+}
+
+__global__ void k_reset_ctl(uint32_t* __restrict__ ctl, uint32_t first, uint32_t count) {
+    if (threadIdx.x < count) ctl[first + threadIdx.x] = 0u;
+}
+
+// ----------------------------------------------------------------------------
+// K2: root bounds (root_bounds, lib.rs:137-154), Morton keys (flat.rs:224-231),
+// per-pass digit histograms, look-back flag reset, per-depth thresholds.
+// ----------------------------------------------------------------------------
+template <uint32_t kKPT>
+__global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict__ sims,
+                                                       const TileRef* __restrict__ stiles,
+                                                       uint32_t* __restrict__ sflags,
+                                                       uint32_t nstiles) {
+    constexpr uint32_t kSortTile = kSortThreads * kKPT;
+    __shared__ float s_red[4][kSortThreads / 64];
+    __shared__ double s_root[3];
+    __shared__ uint32_t s_hist[4][kRadix];
+    const TileRef tr = stiles[blockIdx.x];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t n = S.n;
+    // reset this tile's look-back flags for the four radix passes
+    for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
+        uint32_t pass = k / kRadix, d = k % kRadix;
+        sflags[((size_t)pass * nstiles + blockIdx.x) * kRadix + d] = 0u;
+        s_hist[pass][d] = 0u;
+    }
+    // global f32 min/max of the simulation from the bounds partials
+    const uint32_t nparts = (n + kTile - 1) / kTile;
+    float mnx = __builtin_inff(), mny = __builtin_inff(), mxx = -__builtin_inff(), mxy = -__builtin_inff();
+    for (uint32_t k = threadIdx.x; k < nparts; k += kSortThreads) {
+        float4 b = S.bpart[k];
+        mnx = fminf(mnx, b.x);
+        mny = fminf(mny, b.y);
+        mxx = fmaxf(mxx, b.z);
+        mxy = fmaxf(mxy, b.w);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, off));
+        mny = fminf(mny, __shfl_xor(mny, off));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+    }
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        s_red[0][w] = mnx;
+        s_red[1][w] = mny;
+        s_red[2][w] = mxx;
+        s_red[3][w] = mxy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t k = 1; k < kSortThreads / 64; ++k) {
+            mnx = fminf(mnx, s_red[0][k]);
+            mny = fminf(mny, s_red[1][k]);
+            mxx = fmaxf(mxx, s_red[2][k]);
+            mxy = fmaxf(mxy, s_red[3][k]);
+        }
+        // root_bounds in f64 over the widened f32 positions (min/max are exact)
+        double min_x = mnx, min_y = mny, max_x = mxx, max_y = mxy;
+        double sx = max_x - min_x, sy = max_y - min_y;
+        double span = sy > sx ? sy : sx;
+        span = 1e-12 > span ? 1e-12 : span;
+        s_root[0] = (min_x + max_x) * 0.5;
+        s_root[1] = (min_y + max_y) * 0.5;
+        s_root[2] = span * 0.500000000001 + 1e-12;
+    }
+    __syncthreads();
+    const double cx = s_root[0], cy = s_root[1], half = s_root[2];
+    if (tr.base == 0 && threadIdx.x < 64) {
+        SimMeta* M = S.meta;
+        if (threadIdx.x == 0) {
+            M->root[0] = cx;
+            M->root[1] = cy;
+            M->root[2] = half;
+            M->root[3] = 0.0;
+            M->bounds_f32[0] = (float)cx;
+            M->bounds_f32[1] = (float)cy;
+            M->bounds_f32[2] = (float)half;
+            M->bounds_f32[3] = 0.0f;
+            M->nspan = 0u;
+            M->theta_bits = __float_as_uint(S.theta);
+        }
+        if (threadIdx.x < kLevels) {
+            // half at depth d is half * 2^-d exactly in f64; one rounding to f32
+            double hd = half;
+            for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
+            M->T[threadIdx.x] = accept_threshold((float)hd, S.theta);
+        }
+    }
+    // Morton keys of this tile
+#pragma unroll
+    for (uint32_t j = 0; j < kKPT; ++j) {
+        uint32_t i = tr.base + j * kSortThreads + threadIdx.x;
+        if (i < n) {
+            float2 p = S.pos[i];
+            uint32_t code = spread16(quantize_axis((double)p.x, cx, half)) |
+                            (spread16(quantize_axis((double)p.y, cy, half)) << 1);
+            S.key0[i] = code;
+            S.val0[i] = i;
+            atomicAdd(&s_hist[0][code & 255u], 1u);
+            atomicAdd(&s_hist[1][(code >> 8) & 255u], 1u);
+            atomicAdd(&s_hist[2][(code >> 16) & 255u], 1u);
+            atomicAdd(&s_hist[3][code >> 24], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
+        uint32_t c = s_hist[k / kRadix][k % kRadix];
+        if (c) atomicAdd(&S.meta->hist[k], c);
+    }
+    (void)kSortTile;
+}
+
+// ----------------------------------------------------------------------------
+// K3: one stable LSD radix pass over 8-bit digits (onesweep: local ranking with
+// wavefront ballots, LDS-staged scatter, decoupled look-back across tiles).
+// Equals the stable sort_by_key of flat.rs:233-234 after four passes.
+// ----------------------------------------------------------------------------
+template <uint32_t kKPT>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __restrict__ sims,
+                                                            const TileRef* __restrict__ stiles,
+                                                            uint32_t nstiles, uint32_t pass,
+                                                            uint32_t* __restrict__ ctl,
+                                                            uint32_t* __restrict__ sflags) {
+    constexpr uint32_t kSortTile = kSortThreads * kKPT;
+    constexpr uint32_t kWaves = kSortThreads / 64;
+    constexpr uint32_t kChunk = 64 * kKPT;  // keys per wavefront
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wcnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];  // tile-local start of each digit bucket
+    __shared__ uint32_t s_glob[kRadix];   // destination of each digit bucket
+    __shared__ uint32_t s_scan[kWaves];
+    __shared__ uint32_t s_ticket;
+
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlSortTicket + pass], 1u);
+    for (uint32_t k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&s_wcnt[0][0])[k] = 0u;
+    __syncthreads();
+    const uint32_t t = s_ticket;
+    const TileRef tr = stiles[t];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t n = S.n;
+    const uint32_t shift = 8u * pass;
+    const uint32_t* __restrict__ kin = (pass & 1u) ? S.key1 : S.key0;
+    const uint32_t* __restrict__ vin = (pass & 1u) ? S.val1 : S.val0;
+    uint32_t* __restrict__ kout = (pass & 1u) ? S.key0 : S.key1;
+    uint32_t* __restrict__ vout = (pass & 1u) ? S.val0 : S.val1;
+    uint32_t* flags = sflags + (size_t)pass * nstiles * kRadix;
+
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    const unsigned long long lt_mask = __lanemask_lt();
+    uint32_t key[kKPT], val[kKPT], rank[kKPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kKPT; ++j) {
+        const uint32_t i = tr.base + w * kChunk + j * 64 + lane;
+        const bool valid = i < n;
+        key[j] = valid ? kin[i] : 0u;
+        val[j] = valid ? vin[i] : 0u;
+        const uint32_t d = (key[j] >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t prior = s_wcnt[w][d];  // read by every peer before the leader writes
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) s_wcnt[w][d] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = valid ? prior + below : 0xffffffffu;
+    }
+    __syncthreads();
+    // per digit: offsets of each wavefront inside the digit bucket, tile total
+    const uint32_t d = threadIdx.x;  // kSortThreads == kRadix
+    uint32_t tile_count = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kWaves; ++k) {
+        uint32_t c = s_wcnt[k][d];
+        s_wcnt[k][d] = tile_count;
+        tile_count += c;
+    }
+    uint32_t total;
+    const uint32_t start = block_excl_sum<kSortThreads>(tile_count, s_scan, total);
+    s_start[d] = start;
+    // publish and look back
+    const bool seg_first = (t == S.first_sort_tile);
+    if (seg_first) {
+        // digit base of this simulation: exclusive scan of its histogram
+        uint32_t tot2;
+        const uint32_t base = block_excl_sum<kSortThreads>(S.meta->hist[pass * kRadix + d], s_scan, tot2);
+        flag_store(flags + (size_t)t * kRadix + d, kFlagInc | (base + tile_count));
+        s_glob[d] = base;
+    } else {
+        flag_store(flags + (size_t)t * kRadix + d, kFlagAgg | tile_count);
+        uint32_t excl = lookback(flags + d, kRadix, t, &ctl[kCtlError]);
+        flag_store(flags + (size_t)t * kRadix + d, kFlagInc | (excl + tile_count));
+        s_glob[d] = excl;
+    }
+    __syncthreads();
+    // LDS-staged reorder: digit buckets become contiguous in LDS
+#pragma unroll
+    for (uint32_t j = 0; j < kKPT; ++j) {
+        if (rank[j] != 0xffffffffu) {
+            const uint32_t dj = (key[j] >> shift) & 255u;
+            const uint32_t lpos = s_start[dj] + s_wcnt[w][dj] + rank[j];
+            s_keys[lpos] = key[j];
+            s_vals[lpos] = val[j];
+        }
+    }
+    __syncthreads();
+    const bool last_pass = (pass == 3);
+    for (uint32_t l = threadIdx.x; l < total; l += kSortThreads) {
+        const uint32_t kk = s_keys[l];
+        const uint32_t vv = s_vals[l];
+        const uint32_t dd = (kk >> shift) & 255u;
+        const uint32_t g = s_glob[dd] + (l - s_start[dd]);
+        kout[g] = kk;
+        vout[g] = vv;
+        if (last_pass) {  // sorted body copy + target positions (bh_parallel_tree_positions)
+            const float2 p = S.pos[vv];
+            S.sbody[g] = make_float4(p.x, p.y, S.mass[vv], 0.0f);
+            S.pos_of[vv] = g;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// K4: cell columns.  Position p starts cells at depths dmin(p)..dleaf(p): dmin
+// from the common prefix with its predecessor, existence from the parent group's
+// size, leaf depth from the key at p + bucket.  An exclusive scan of column
+// lengths (decoupled look-back) gives every cell its preorder index.  Writes
+// ranges, geometry (f64, flat.rs:96-103), first-child links, leaf aggregates
+// (fill_aggregates leaf branch, flat.rs:181-195) and thresholds.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t group_start(const uint32_t* __restrict__ K, uint32_t p,
+                                                uint32_t kp, uint32_t d) {
+    if (d == 0) return 0u;
+    // gallop backward: largest q <= p whose predecessor differs at depth d
+    uint32_t hi = p;       // same prefix
+    uint32_t step = 1;
+    uint32_t lo;           // candidate with different prefix (or -1 => 0)
+    for (;;) {
+        if (step > hi) { lo = 0xffffffffu; break; }
+        uint32_t q = hi - step;
+        if (!same_prefix(K[q], kp, d)) { lo = q; break; }
+        hi = q;
+        step <<= 1;
+    }
+    // answer in (lo, hi]; binary search for the first same-prefix position
+    uint32_t a = (lo == 0xffffffffu) ? 0u : lo + 1u;
+    uint32_t b = hi;
+    while (a < b) {
+        uint32_t mid = a + ((b - a) >> 1);
+        if (same_prefix(K[mid], kp, d)) b = mid; else a = mid + 1;
+    }
+    return a;
+}
+
+__device__ __forceinline__ uint32_t group_end(const uint32_t* __restrict__ K, uint32_t p,
+                                              uint32_t kp, uint32_t d, uint32_t limit) {
+    // first q in (p, limit] whose key differs from kp at depth d; limit <= n
+    if (d == 0) return limit;
+    uint32_t lo = p;  // same prefix
+    uint32_t step = 1;
+    uint32_t hi;
+    for (;;) {
+        uint32_t q = p + step;
+        if (q >= limit) { hi = limit; break; }
+        if (!same_prefix(K[q], kp, d)) { hi = q; break; }
+        lo = q;
+        step <<= 1;
+    }
+    uint32_t a = lo + 1, b = hi;  // first differing in [a, b]
+    while (a < b) {
+        uint32_t mid = a + ((b - a) >> 1);
+        if (same_prefix(K[mid], kp, d)) a = mid + 1; else b = mid;
+    }
+    return a;
+}
+
+__global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restrict__ sims,
+                                                          const TileRef* __restrict__ ttiles,
+                                                          uint32_t* __restrict__ ctl,
+                                                          uint32_t* __restrict__ tflags) {
+    __shared__ uint32_t s_scan[kTopoTile / 64];
+    __shared__ uint32_t s_ticket;
+    __shared__ uint32_t s_excl;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
+    __syncthreads();
+    const uint32_t t = s_ticket;
+    const TileRef tr = ttiles[t];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
+    const uint32_t* __restrict__ K = S.key0;
+    const uint32_t p = tr.base + threadIdx.x;
+    const bool valid = p < n;
+    const uint32_t kp = valid ? K[p] : 0u;
+    uint32_t dmin = 17, count = 0, dleaf = 0, pl = 0;
+    if (valid) {
+        dmin = (p == 0) ? 0u : ((K[p - 1] == kp) ? 17u : lcp2(K[p - 1], kp) + 1u);
+        if (dmin <= maxd) {
+            bool exists = true;
+            if (dmin >= 1) {
+                const uint32_t dd = dmin - 1;
+                pl = group_start(K, p, kp, dd);
+                const uint32_t q = pl + bucket;
+                exists = (q < n) && same_prefix(K[q], kp, dd);
+            }
+            if (exists) {
+                const int L = (p + bucket < n) ? (int)lcp2(kp, K[p + bucket]) : -1;
+                int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
+                if (dl < (int)dmin) dl = (int)dmin;
+                dleaf = (uint32_t)dl;
+                count = dleaf - dmin + 1;
+            }
+        }
+    }
+    uint32_t total;
+    const uint32_t mine = block_excl_sum<kTopoTile>(count, s_scan, total);
+    if (threadIdx.x == 0) {
+        uint32_t* f = tflags + t;
+        uint32_t excl;
+        if (t == S.first_topo_tile) {
+            excl = 0;
+            flag_store(f, kFlagInc | total);
+        } else {
+            flag_store(f, kFlagAgg | total);
+            excl = lookback(tflags, 1, t, &ctl[kCtlError]);
+            flag_store(f, kFlagInc | (excl + total));
+        }
+        s_excl = excl;
+        if (tr.base + kTopoTile >= n) S.meta->ncells = excl + total;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const uint32_t base = s_excl + mine;
+    S.cbase[p] = base;
+    S.cdmin[p] = (uint8_t)dmin;
+    S.ccount[p] = (uint8_t)count;
+    S.cparent[p] = pl;
+    if (count == 0) return;
+    if (base + count > S.cell_cap) {
+        atomicOr(&S.meta->err, kErrCellCapacity);
+        return;
+    }
+    // geometry at dmin along the key's quadrant path (child_bounds, flat.rs:96-103)
+    const SimMeta* M = S.meta;
+    double cx = M->root[0], cy = M->root[1], h = M->root[2];
+    for (uint32_t l = 0; l < dmin; ++l) {
+        const uint32_t q = (kp >> (30u - 2u * l)) & 3u;
+        h = h * 0.5;
+        cx = cx + ((q & 1u) ? h : -h);
+        cy = cy + ((q & 2u) ? h : -h);
+    }
+    uint32_t limit = n;
+    for (uint32_t d = dmin; d <= dleaf; ++d) {
+        const uint32_t idx = base + (d - dmin);
+        const uint32_t e = group_end(K, p, kp, d, limit);
+        limit = e;
+        const bool leaf = (d == dleaf);
+        CellT c;
+        c.start = p;
+        c.end = e;
+        c.skip = 0u;
+        c.info = d | (leaf ? kInfoLeaf : 0u);
+        c.T = M->T[d];
+        uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
+        if (leaf) {
+            double mass = 0.0, wx = 0.0, wy = 0.0;
+            for (uint32_t q = p; q < e; ++q) {
+                const float4 b = S.sbody[q];
+                const double bm = b.z;
+                mass = mass + bm;
+                wx = wx + bm * (double)b.x;
+                wy = wy + bm * (double)b.y;
+            }
+            const double comx = wx / mass, comy = wy / mass;
+            S.cagg[idx] = Agg{mass, comx, comy, 0.0};
+            c.comx = (float)comx;
+            c.comy = (float)comy;
+            c.mass = (float)mass;
+        } else {
+            const uint32_t q0 = (kp >> (30u - 2u * d)) & 3u;
+            (&ch.x)[q0] = idx + 1u;
+            c.comx = 0.f;
+            c.comy = 0.f;
+            c.mass = 0.f;
+        }
+        S.cells[idx] = c;
+        S.cchild[idx] = ch;
+        S.cgeom[idx] = make_float4((float)cx, (float)cy, (float)h, 0.0f);
+        if (!leaf) {
+            const uint32_t q = (kp >> (30u - 2u * d)) & 3u;
+            h = h * 0.5;
+            cx = cx + ((q & 1u) ? h : -h);
+            cy = cy + ((q & 2u) ? h : -h);
+        }
+    }
+}
+
+// Internal-cell aggregate from its children in quadrant order
+// (fill_aggregates internal branch, flat.rs:196-213).
+__device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, uint4 ch) {
+    double mass = 0.0, wx = 0.0, wy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t c = (&ch.x)[q];
+        if (c != kNoChild) {
+            const Agg a = cagg[c];
+            mass = mass + a.m;
+            wx = wx + a.m * a.x;
+            wy = wy + a.m * a.y;
+        }
+    }
+    return Agg{mass, wx / mass, wy / mass, 0.0};
+}
+
+// ----------------------------------------------------------------------------
+// K5: skip links (next cell after each subtree), the remaining child links, and
+// bottom-up aggregates of every internal cell whose range lies inside this tile.
+// Internal cells that cross a tile boundary go to the simulation's span list.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
+                                                         const TileRef* __restrict__ ttiles) {
+    const TileRef tr = ttiles[blockIdx.x];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t n = S.n;
+    const uint32_t p = tr.base + threadIdx.x;
+    const bool valid = p < n;
+    const uint32_t ncells = S.meta->ncells;
+    uint32_t count = 0, dmin = 17, base = 0;
+    if (valid) {
+        count = S.ccount[p];
+        dmin = S.cdmin[p];
+        base = S.cbase[p];
+    }
+    if (count > 0 && base + count <= S.cell_cap) {
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t e = S.cells[base + k].end;
+            S.cells[base + k].skip = e < n ? S.cbase[e] : ncells;
+        }
+        if (dmin >= 1) {
+            const uint32_t pl = S.cparent[p];
+            const uint32_t parent = S.cbase[pl] + (dmin - 1u - S.cdmin[pl]);
+            const uint32_t kp = S.key0[p];
+            const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
+            if (S.cdmin[pl] <= dmin - 1u && parent < ncells) (&S.cchild[parent].x)[q] = base;
+            else atomicOr(&S.meta->err, kErrCellCapacity);
+        }
+    } else {
+        count = 0;
+    }
+    __syncthreads();
+    const uint32_t tile_end = tr.base + kTopoTile;
+    const uint32_t dleaf = dmin + count - 1u;
+    for (int d = 15; d >= 0; --d) {
+        if (count > 0 && (uint32_t)d >= dmin && (uint32_t)d < dleaf) {
+            const uint32_t idx = base + ((uint32_t)d - dmin);
+            const uint32_t e = S.cells[idx].end;
+            if (e <= tile_end) {
+                const Agg a = combine_children(S.cagg, S.cchild[idx]);
+                S.cagg[idx] = a;
+                S.cells[idx].comx = (float)a.x;
+                S.cells[idx].comy = (float)a.y;
+                S.cells[idx].mass = (float)a.m;
+            } else {
+                const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
+                if (slot < S.span_cap) {
+                    S.span[slot] = idx;
+                    S.cells[idx].info |= (slot + 1u) << kInfoSpanShift;
+                } else {
+                    atomicOr(&S.meta->err, kErrSpanCapacity);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------
+// K6: aggregates of the spanning internal cells (the top of the tree), one
+// workgroup per simulation, bottom-up by depth with the running values in LDS.
+// ----------------------------------------------------------------------------
+constexpr uint32_t kTopThreads = 1024;
+
+__global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims) {
+    __shared__ double s_m[kSpanSlotsLds], s_x[kSpanSlotsLds], s_y[kSpanSlotsLds];
+    __shared__ uint8_t s_depth[kSpanSlotsLds];
+    const SimDesc& S = sims[blockIdx.x];
+    const uint32_t nspan = min(S.meta->nspan, S.span_cap);
+    if (nspan == 0) return;
+    const bool in_lds = nspan <= kSpanSlotsLds;
+    for (uint32_t i = threadIdx.x; i < nspan && in_lds; i += kTopThreads)
+        s_depth[i] = (uint8_t)(S.cells[S.span[i]].info & 255u);
+    __syncthreads();
+    for (int d = 15; d >= 0; --d) {
+        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+            const uint32_t idx = S.span[i];
+            const uint32_t depth = in_lds ? s_depth[i] : (S.cells[idx].info & 255u);
+            if (depth != (uint32_t)d) continue;
+            const uint4 ch = S.cchild[idx];
+            double mass = 0.0, wx = 0.0, wy = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = (&ch.x)[q];
+                if (c == kNoChild) continue;
+                double cm, cxx, cyy;
+                const uint32_t slot = S.cells[c].info >> kInfoSpanShift;
+                if (in_lds && slot != 0u) {
+                    cm = s_m[slot - 1];
+                    cxx = s_x[slot - 1];
+                    cyy = s_y[slot - 1];
+                } else {
+                    const Agg a = S.cagg[c];
+                    cm = a.m;
+                    cxx = a.x;
+                    cyy = a.y;
+                }
+                mass = mass + cm;
+                wx = wx + cm * cxx;
+                wy = wy + cm * cyy;
+            }
+            const double comx = wx / mass, comy = wy / mass;
+            if (in_lds) {
+                s_m[i] = mass;
+                s_x[i] = comx;
+                s_y[i] = comy;
+            }
+            S.cagg[idx] = Agg{mass, comx, comy, 0.0};
+            S.cells[idx].comx = (float)comx;
+            S.cells[idx].comy = (float)comy;
+            S.cells[idx].mass = (float)mass;
+        }
+        __syncthreads();
+    }
+}
+
+// Thresholds for a new theta (force called with a theta other than the build's).
+__global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
+    const uint32_t s = blockIdx.x;
+    if (s >= nsims) return;
+    const SimDesc& S = sims[s];
+    if (threadIdx.x < kLevels) {
+        double hd = S.meta->root[2];
+        for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
+        S.meta->T[threadIdx.x] = accept_threshold((float)hd, S.theta);
+    }
+    if (threadIdx.x == 0) S.meta->theta_bits = __float_as_uint(S.theta);
+}
+
+__global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
+    const SimDesc& S = sims[sim];
+    const uint32_t nc = S.meta->ncells;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
+        S.cells[i].T = S.meta->T[S.cells[i].info & 255u];
+}
+
+// ----------------------------------------------------------------------------
+// K7: force traversal (bh_traverse, nbody.wgsl:43-123), stackless over the
+// preorder cells.  A wavefront owns 64 consecutive sorted targets and walks the
+// union of their DFS sequences with one wave-uniform cursor; each lane acts
+// only on the cells of its own sequence, so its interaction list and f32
+// summation order are exactly the per-target DFS of the reference.  Work is
+// pulled from a global queue of (simulation, 64-target tile) items.
+// ----------------------------------------------------------------------------
+template <bool kEvidence>
+__global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ sims,
+                                                  const TileRef* __restrict__ wtiles,
+                                                  uint32_t nwtiles, uint32_t* __restrict__ ctl,
+                                                  uint32_t* __restrict__ ev_stats,
+                                                  unsigned long long* __restrict__ ev_hash,
+                                                  uint32_t ev_sim) {
+    const uint32_t lane = lane_id();
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(&ctl[kCtlQueue], 1u);
+        item = __builtin_amdgcn_readlane(item, 0);
+        if (item >= nwtiles) break;
+        const TileRef tr = wtiles[item];
+        const SimDesc& S = sims[tr.sim];
+        const uint32_t n = S.n;
+        const uint32_t p = tr.base + lane;
+        const bool valid = p < n;
+        const CellT* __restrict__ cl = S.cells;
+        const float4* __restrict__ sb = S.sbody;
+        const uint32_t ncells = S.meta->ncells;
+        const float eps = S.eps;
+        const float eps2 = eps * eps;
+        const float G = S.G;
+        float tx = 0.f, ty = 0.f;
+        if (valid) {
+            const float4 b = sb[p];
+            tx = b.x;
+            ty = b.y;
+        }
+        float ax = 0.f, ay = 0.f;
+        uint32_t nxt = valid ? 0u : 0xffffffffu;
+        uint32_t visited = 0, accepted = 0, direct = 0;
+        unsigned long long h = 0xcbf29ce484222325ull;
+        uint32_t c = 0;
+        while (c < ncells) {
+            const CellT cell = cl[c];
+            const bool act = (nxt == c);
+            // the cursor only moves forward (skip > c in a well-formed tree), so
+            // the walk ends after at most ncells iterations
+            const uint32_t skip = max(cell.skip, c + 1);
+            if (cell.info & kInfoLeaf) {
+                if (act) {
+                    const uint32_t qend = min(cell.end, n);
+                    for (uint32_t q = cell.start; q < qend; ++q) {
+                        if (q != p) {
+                            const float4 b = sb[q];
+                            point_mass(tx, ty, b.x, b.y, b.z, eps2, G, ax, ay);
+                            if (kEvidence) {
+                                ++direct;
+                                mix64(h, (1ull << 60) | (unsigned long long)S.val0[q]);
+                            }
+                        }
+                    }
+                    nxt = skip;
+                    if (kEvidence) ++visited;
+                }
+            } else if (act) {
+                const float dx = cell.comx - tx;
+                const float dy = cell.comy - ty;
+                const float dd = dx * dx;
+                const float ee = dy * dy;
+                const float d2 = dd + ee;
+                const bool contains = (p - cell.start) < (cell.end - cell.start);
+                if (kEvidence) ++visited;
+                if (!contains && d2 >= cell.T) {
+                    point_mass(tx, ty, cell.comx, cell.comy, cell.mass, eps2, G, ax, ay);
+                    nxt = skip;
+                    if (kEvidence) {
+                        ++accepted;
+                        mix64(h, (2ull << 60) | ((unsigned long long)(cell.info & 255u) << 32) |
+                                     cell.start);
+                    }
+                } else {
+                    nxt = c + 1;
+                }
+            }
+            if (__ballot(nxt == c + 1)) c = c + 1;
+            else c = wave_min_u32(nxt);
+        }
+        if (valid) {
+            const uint32_t body = S.val0[p];
+            S.acc[body] = make_float2(ax, ay);
+            if (kEvidence && tr.sim == ev_sim) {
+                if (ev_stats) {
+                    ev_stats[3 * (size_t)body + 0] = visited;
+                    ev_stats[3 * (size_t)body + 1] = accepted;
+                    ev_stats[3 * (size_t)body + 2] = direct;
+                }
+                if (ev_hash) ev_hash[body] = h;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// K8: final kick (bh_final_kick, nbody_evolve.wgsl:35-46)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kTile) void k_final_kick(const SimDesc* __restrict__ sims,
+                                                      const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t i = tr.base + threadIdx.x;
+    if (i >= S.n) return;
+    const float h = 0.5f * S.dt;
+    float2 v = S.vel[i];
+    const float2 a = S.acc[i];
+    v.x = v.x + h * a.x;
+    v.y = v.y + h * a.y;
+    S.vel[i] = v;
+}
+
+// bh_kick_drift alone (nbody_evolve.wgsl:19-33), for the drop-in kick_drift call
+__global__ __launch_bounds__(kTile) void k_kick_drift(const SimDesc* __restrict__ sims,
+                                                      const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    const SimDesc& S = sims[tr.sim];
+    const uint32_t i = tr.base + threadIdx.x;
+    if (i >= S.n) return;
+    const float dt = S.dt;
+    const float h = 0.5f * dt;
+    float2 v = S.vel[i];
+    float2 p = S.pos[i];
+    const float2 a = S.acc[i];
+    const float hvx = v.x + h * a.x;
+    const float hvy = v.y + h * a.y;
+    p.x = p.x + dt * hvx;
+    p.y = p.y + dt * hvy;
+    S.pos[i] = p;
+    S.vel[i] = make_float2(hvx, hvy);
+}
+
+// ABI <-> SoA conversion of the frozen records
+__global__ void k_unpack_state(const GbhEvolveState* __restrict__ in, uint32_t n, float2* pos,
+                               float* mass, float2* vel, float2* acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GbhEvolveState s = in[i];
+    pos[i] = make_float2(s.position_mass[0], s.position_mass[1]);
+    mass[i] = s.position_mass[2];
+    vel[i] = make_float2(s.velocity[0], s.velocity[1]);
+    acc[i] = make_float2(0.f, 0.f);
+}
+
+__global__ void k_pack_state(GbhEvolveState* __restrict__ out, uint32_t n, const float2* pos,
+                             const float* mass, const float2* vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    GbhEvolveState s;
+    s.position_mass[0] = pos[i].x;
+    s.position_mass[1] = pos[i].y;
+    s.position_mass[2] = mass[i];
+    s.position_mass[3] = 0.f;
+    s.velocity[0] = vel[i].x;
+    s.velocity[1] = vel[i].y;
+    s.velocity[2] = 0.f;
+    s.velocity[3] = 0.f;
+    out[i] = s;
+}
+
+__global__ void k_pack_accel(GbhGpuAccel* __restrict__ out, uint32_t n, const float2* acc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    GbhGpuAccel a;
+    a.value[0] = acc[i].x;
+    a.value[1] = acc[i].y;
+    a.value[2] = 0.f;
+    a.value[3] = 0.f;
+    out[i] = a;
+}
+
+}  // namespace gbh

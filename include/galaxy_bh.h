@@ -1,0 +1,278 @@
+/*
+ * galaxy_bh.h — C ABI of the MI355X (gfx950) Barnes-Hut evolution library.
+ *
+ * Drop-in boundary for GALAXY's BH #2D resident Barnes-Hut step.  Every entry
+ * point below names the reference interface it replaces (file:line in the
+ * GALAXY repository).  Conventions shared by all entry points:
+ *   - return 0 on success, nonzero on failure; the failure text is available
+ *     from gbh_last_error() on the calling thread (thread-local), matching the
+ *     human-readable Box<dyn Error> strings of runtime/src/config.rs:7;
+ *   - calls are synchronous: they return after the GPU work they issued has
+ *     completed, like the reference's submit + device.poll(Wait);
+ *   - one host thread drives a device handle at a time;
+ *   - only plain pointers and sizes cross the boundary.
+ */
+#ifndef GALAXY_BH_H
+#define GALAXY_BH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBH_NO_CHILD 0xffffffffu
+/* nbody_parallel.rs:20 (MAX_PARALLEL_TREE_BODIES) */
+#define GBH_MAX_PARALLEL_TREE_BODIES 65536u
+/* Upper bound of the beyond-cap mode (gbh_tree_create_large). */
+#define GBH_MAX_LARGE_TREE_BODIES (1u << 28)
+
+/* ---------------------------------------------------------------------------
+ * Frozen f32/u32 transfer records, byte-identical to runtime/src/nbody_gpu.rs:22-83
+ * (sizes pinned at nbody_gpu.rs:1441-1451, 144-byte fixture at :1453-1489).
+ * ------------------------------------------------------------------------- */
+typedef struct { uint32_t info[4]; float physics[4]; } GbhBhSettings;      /* 32 B: {N, cells, root, 0} {theta, eps, G, 0} */
+typedef struct { float position_mass[4]; uint32_t meta[4]; } GbhGpuBody;  /* 32 B: {x, y, m, 0} {sorted position, 0, 0, 0} */
+typedef struct { uint32_t data[4]; } GbhGpuEntry;                         /* 16 B: {Morton code, body index, 0, 0} */
+typedef struct {
+    float center_half_mass[4]; /* cx, cy, half, mass */
+    float com[4];              /* com_x, com_y, 0, 0 */
+    uint32_t range_depth[4];   /* start, end, depth, valid */
+    uint32_t children[4];      /* child ids or GBH_NO_CHILD */
+} GbhGpuCell;                                                             /* 64 B */
+typedef struct { float value[4]; } GbhGpuAccel;                           /* 16 B: {ax, ay, 0, 0} */
+typedef struct { uint32_t info[4]; float motion[4]; } GbhEvolveSettings;  /* 32 B: {N, 0, 0, 0} {dt, 0, 0, 0} */
+typedef struct { float position_mass[4]; float velocity[4]; } GbhEvolveState; /* 32 B */
+typedef struct { uint32_t info[4]; float physics[4]; } GbhTreeSettings;   /* 32 B */
+typedef struct { uint32_t data[4]; float bounds[4]; } GbhTreeMeta;        /* 32 B: data 0; {cx, cy, half, 0} */
+
+/* galaxy_nbody::Config (nbody/src/lib.rs:27-46); validated as Config::validate (:48-67). */
+typedef struct {
+    double theta;
+    double softening;
+    double g;
+    uint32_t bucket;
+    uint32_t max_depth;
+} gbh_config;
+
+/* StageTiming (nbody_gpu.rs:125-130) */
+typedef struct {
+    double transfer_seconds;
+    double dispatch_seconds;
+    double readback_seconds;
+} gbh_stage_timing;
+
+/* ParallelTreeBuildTiming (nbody_parallel.rs:45-53): host wall time per stage
+ * family; stages that the GPU library fuses report their time in the stage that
+ * launches them. */
+typedef struct {
+    double bounds_seconds;
+    double morton_seconds;
+    double radix_seconds;
+    double positions_seconds;
+    double topology_seconds;
+    double aggregate_seconds;
+} gbh_build_timing;
+
+typedef struct gbh_device gbh_device;
+typedef struct gbh_state gbh_state;
+typedef struct gbh_runtime gbh_runtime;
+typedef struct gbh_tree gbh_tree;
+typedef struct gbh_batch gbh_batch;
+typedef struct gbh_server gbh_server;
+
+/* Thread-local text of the last failure (runtime/src/config.rs:7 error strings). */
+const char* gbh_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Device (NbodyGpu::new, nbody_gpu.rs:331-621; select_adapter :278-296)
+ * selector: NULL, a decimal device index, or a case-insensitive name substring.
+ * allow_software is accepted for interface parity; ROCm exposes no software
+ * adapter, so "software" is always false.
+ * ------------------------------------------------------------------------- */
+int gbh_device_open(const char* selector, int allow_software, gbh_device** out);
+void gbh_device_close(gbh_device* dev);
+/* describe()/info() (nbody_gpu.rs:298-310, 782-784): JSON object with keys
+ * index, adapter_count, name, backend, device_type, driver, driver_info, software.
+ * Writes at most cap bytes (NUL-terminated) and the full length to *len. */
+int gbh_device_info_json(gbh_device* dev, char* buf, size_t cap, size_t* len);
+
+/* ---------------------------------------------------------------------------
+ * Persistent evolving state (EvolvingState, nbody_gpu.rs:132-138)
+ * ------------------------------------------------------------------------- */
+/* create_evolving_state (nbody_gpu.rs:786-852): n in 2..=u32::MAX, finite values,
+ * mass > 0 ("body {i} contains invalid evolving state"). */
+int gbh_state_create(gbh_device* dev, const GbhEvolveState* states, uint32_t n,
+                     gbh_state** out, gbh_stage_timing* timing);
+void gbh_state_destroy(gbh_state* st);
+uint32_t gbh_state_count(const gbh_state* st);
+/* evolving_buffer_bytes (nbody_gpu.rs:1116-1118) */
+uint64_t gbh_state_bytes(const gbh_state* st);
+/* read_evolving_state (nbody_gpu.rs:1019-1069): rejects non-finite or mass <= 0. */
+int gbh_state_read(gbh_state* st, GbhEvolveState* out, gbh_stage_timing* timing);
+/* read_evolving_accelerations (nbody_gpu.rs:1071-1114): rejects non-finite. */
+int gbh_accel_read(gbh_state* st, GbhGpuAccel* out, gbh_stage_timing* timing);
+/* kick_drift / final_kick (nbody_gpu.rs:1011-1017, integrate_stage :940-1009):
+ * dt finite and > 0, applied as f32. */
+int gbh_kick_drift(gbh_state* st, double dt, gbh_stage_timing* timing);
+int gbh_final_kick(gbh_state* st, double dt, gbh_stage_timing* timing);
+
+/* ---------------------------------------------------------------------------
+ * BH #2D parallel tree (ParallelTreeRuntime, nbody_parallel.rs:93-993)
+ * ------------------------------------------------------------------------- */
+/* ParallelTreeRuntime::new (nbody_parallel.rs:156-307) */
+int gbh_runtime_create(gbh_device* dev, gbh_runtime** out);
+void gbh_runtime_destroy(gbh_runtime* rt);
+/* create_tree (nbody_parallel.rs:361-493): 2..=65,536 resident bodies. */
+int gbh_tree_create(gbh_runtime* rt, gbh_state* st, gbh_tree** out);
+/* Beyond-cap mode (no reference counterpart): 2..=GBH_MAX_LARGE_TREE_BODIES,
+ * compact cell layout; same kernels and results contract. */
+int gbh_tree_create_large(gbh_runtime* rt, gbh_state* st, gbh_tree** out);
+void gbh_tree_destroy(gbh_tree* tree);
+/* rebuild (nbody_parallel.rs:555-826) */
+int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* cfg,
+                     gbh_build_timing* timing);
+/* force (nbody_parallel.rs:828-897): writes the state's acceleration buffer. */
+int gbh_force(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* cfg,
+              gbh_stage_timing* timing);
+/* Force with per-target interaction evidence (no reference counterpart; the
+ * parity check of the traversal): stats[3*i] = {visited cells, accepted cells,
+ * direct terms} and hash[i] = FNV-mix64 over body i's ordered interaction list
+ * (direct term: (1<<60)|source body; accepted cell: (2<<60)|(depth<<32)|start).
+ * Either pointer may be NULL. */
+int gbh_force_evidence(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* cfg,
+                       uint32_t* stats, uint64_t* hash);
+/* Cell statistics of the last rebuild: active cells, leaves, max depth. */
+int gbh_tree_counts(gbh_tree* tree, uint32_t* active, uint32_t* leaves, uint32_t* max_depth);
+/* read_evidence (nbody_parallel.rs:929-988): sparse level-order layout, cells in
+ * slot order (slot = depth*N + start), children as slot ids, valid = 1,
+ * meta.data = 0.  cells must hold cell_capacity records (>= active). */
+int gbh_tree_read_evidence(gbh_tree* tree, GbhTreeMeta* meta, GbhGpuEntry* entries,
+                           GbhGpuCell* cells, uint32_t cell_capacity, uint32_t* active,
+                           uint32_t* leaves, uint32_t* max_depth);
+/* BH #2A-layout readback (no reference counterpart): the tree in flat.rs preorder,
+ * packed like pack_flat_tree (nbody_gpu.rs:175-250, range_depth[3] = 0), and the
+ * sorted position of every body (entry_position_by_body, flat.rs:236-239). */
+int gbh_tree_read_flat(gbh_tree* tree, GbhGpuEntry* entries, GbhGpuCell* cells,
+                       uint32_t cell_capacity, uint32_t* ncells, uint32_t* position_by_body);
+/* GPU time (ms, HIP events on the library's stream) of the traversal kernel of
+ * the last gbh_force on this tree (no reference counterpart; profiling). */
+double gbh_tree_last_force_ms(gbh_tree* tree);
+/* buffer_bytes (nbody_parallel.rs:990-992): bytes this library actually holds. */
+uint64_t gbh_tree_buffer_bytes(gbh_tree* tree);
+/* The reference's sparse-layout size for n bodies (nbody_parallel.rs:485-491),
+ * kept for receipt-field compatibility. */
+uint64_t gbh_sparse_layout_bytes(uint32_t n);
+
+/* ---------------------------------------------------------------------------
+ * BH #2B1 / #2B2 host-built tree ingest (nbody_gpu.rs:640-780, 854-938)
+ * ------------------------------------------------------------------------- */
+/* traverse (nbody_gpu.rs:640-780): packed records in, accelerations out. */
+int gbh_traverse_packed(gbh_device* dev, const GbhBhSettings* settings, const GbhGpuBody* bodies,
+                        const GbhGpuEntry* entries, const GbhGpuCell* cells, GbhGpuAccel* out,
+                        gbh_stage_timing* timing);
+/* force_into (nbody_gpu.rs:854-938): packed records in, state acceleration out. */
+int gbh_force_packed(gbh_state* st, const GbhBhSettings* settings, const GbhGpuBody* bodies,
+                     const GbhGpuEntry* entries, const GbhGpuCell* cells,
+                     gbh_stage_timing* timing);
+
+/* ---------------------------------------------------------------------------
+ * Batched ensembles (no reference counterpart): many independent simulations of
+ * uneven N advanced together, one launch per phase, whole step captured in a
+ * HIP graph.  Each simulation's results are bit-identical to the same
+ * simulation run alone through the drop-in entry points.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    gbh_config config;
+    double dt;
+} gbh_sim_params;
+
+typedef struct {
+    uint32_t sort_tile;      /* keys per sort workgroup: 1024 or 2048 */
+    uint32_t traverse_block; /* traversal workgroup size: 64, 128, 256 */
+    uint32_t traverse_waves_per_cu; /* persistent traversal occupancy target */
+    uint32_t use_graph;      /* capture each step in a HIP graph */
+} gbh_tuning;
+
+/* counts[nsims]; states = all simulations' records concatenated in order. */
+int gbh_batch_create(gbh_device* dev, uint32_t nsims, const uint32_t* counts,
+                     const GbhEvolveState* states, const gbh_sim_params* params,
+                     const gbh_tuning* tuning, gbh_batch** out);
+void gbh_batch_destroy(gbh_batch* b);
+/* Advance every simulation by `steps` KDK steps (the first call also performs
+ * the initial build + force, like galaxy-bh-gpu-tree-parallel.rs:338-339). */
+int gbh_batch_step(gbh_batch* b, uint32_t steps);
+int gbh_batch_read(gbh_batch* b, uint32_t sim, GbhEvolveState* out, GbhGpuAccel* accel);
+int gbh_batch_force_evidence(gbh_batch* b, uint32_t sim, uint32_t* stats, uint64_t* hash);
+uint64_t gbh_batch_device_bytes(gbh_batch* b);
+/* Kernel time of the last gbh_batch_step per phase (ms, HIP events), for profiling. */
+int gbh_batch_last_phase_ms(gbh_batch* b, double* ms, uint32_t cap, uint32_t* count);
+
+/* ---------------------------------------------------------------------------
+ * Serving (no reference counterpart): continuous batching of "advance this
+ * galaxy k steps" requests with HBM-resident state and HBM-budget admission.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint64_t hbm_budget_bytes;  /* admission budget for resident request state */
+    uint32_t steps_per_round;   /* steps every in-flight request advances per round */
+    uint32_t max_inflight;      /* cap on concurrently resident requests (0 = none) */
+    gbh_tuning tuning;
+} gbh_server_options;
+
+typedef struct {
+    uint64_t id;
+    double submit_time;   /* seconds, server clock */
+    double admit_time;
+    double finish_time;
+    uint32_t n;
+    uint32_t steps;
+    int32_t status;       /* 0 = ok, nonzero = failed (text in gbh_server_error) */
+} gbh_completion;
+
+/* Device bytes one resident simulation of n bodies holds (state + tree buffers);
+ * the unit of HBM-budget admission. */
+uint64_t gbh_sim_device_bytes(uint32_t n);
+int gbh_server_create(gbh_device* dev, const gbh_server_options* opt, gbh_server** out);
+void gbh_server_destroy(gbh_server* s);
+/* Server clock in seconds (monotonic, starts at create). */
+double gbh_server_now(gbh_server* s);
+/* Queue a request; the initial state is copied.  Refused immediately (nonzero)
+ * if the request alone exceeds the HBM budget. */
+int gbh_server_submit(gbh_server* s, const GbhEvolveState* states, uint32_t n,
+                      const gbh_sim_params* params, uint32_t steps, uint64_t* id);
+/* One serving round: admit queued requests while budget and slots allow, advance
+ * every in-flight request, retire finished ones.  Returns completions. */
+int gbh_server_round(gbh_server* s, gbh_completion* done, uint32_t cap, uint32_t* ndone);
+/* Final state of a completed request (then released). */
+int gbh_server_take_result(gbh_server* s, uint64_t id, GbhEvolveState* out, GbhGpuAccel* accel);
+/* queued, in-flight, resident bytes */
+int gbh_server_stats(gbh_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* resident);
+
+/* ---------------------------------------------------------------------------
+ * Seeded synthetic initial conditions (host side; no GPU needed).
+ * The f32 records are the canonical state; generation is f64 then one rounding.
+ * ------------------------------------------------------------------------- */
+/* Exponential disc Sigma ~ exp(-R/scale) truncated at `truncation`*scale, uniform
+ * angles, circular speed from the enclosed mass with softened radius plus
+ * `dispersion` x v_c Gaussian components; equal masses total_mass/n. */
+int gbh_ic_rotating_disc(uint32_t n, uint64_t seed, double total_mass, double scale,
+                         double truncation, double softening, double g, double dispersion,
+                         GbhEvolveState* out);
+/* Uniform unit-radius disc (R = radius*sqrt(u)), zero velocity, equal masses. */
+int gbh_ic_uniform_collapse(uint32_t n, uint64_t seed, double total_mass, double radius,
+                            GbhEvolveState* out);
+/* Two exponential discs on a parabolic approach (COM frame). */
+int gbh_ic_binary_disc(uint32_t n1, uint32_t n2, uint64_t seed1, uint64_t seed2, double m1,
+                       double m2, double separation, double pericentre, double softening,
+                       double g, double dispersion, GbhEvolveState* out);
+/* make_disc / make_collision presets (nbody/src/lib.rs:454-528), kpc/Msun/Myr;
+ * f64 bodies {x, y, vx, vy, mass} are written to `bodies` (5 doubles each). */
+int gbh_ic_reference_disc(uint32_t n, uint64_t seed, double total_mass, double scale_kpc,
+                          double* bodies);
+int gbh_ic_reference_collision(uint32_t n, uint64_t seed, double* bodies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,100 @@
+"""GPU tests of the batched and served forms: every simulation's result is
+bit-identical to the same simulation run alone (oracle trajectory = solo drop-in
+trajectory, which test_gpu_dropin pins), whatever else shares the launches."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from galaxy_amd import Config, NbodyGpu, ics
+from galaxy_amd.batch import Batch
+from galaxy_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    g = NbodyGpu()
+    yield g
+    g.close()
+
+
+def ocfg(oracle, c: Config):
+    return oracle.make_config(theta=c.theta, softening=c.softening_kpc, g=c.g, bucket=c.bucket,
+                              max_depth=c.max_depth)
+
+
+def mixed_sims():
+    disc = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    coll = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)
+    return [
+        (ics.rotating_disc(700, 1001, scale=0.7), disc, 1e-3),
+        (ics.uniform_collapse(5000, 1002), coll, 5e-4),
+        (ics.rotating_disc(33, 1003), disc, 1e-3),
+        (ics.rotating_disc(2049, 1004, scale=1.8), Config(theta=0.6, softening_kpc=0.01, g=1.0,
+                                                          bucket=1, max_depth=16), 1e-3),
+        (ics.uniform_collapse(64, 1005), coll, 5e-4),
+    ]
+
+
+@pytest.mark.parametrize("use_graph,sort_tile", [(True, 2048), (False, 1024)])
+def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile):
+    sims = mixed_sims()
+    print(f"batch graph={use_graph} sort_tile={sort_tile}", flush=True)
+    b = Batch(gpu, [s for s, _, _ in sims], [c for _, c, _ in sims], [dt for _, _, dt in sims],
+              use_graph=use_graph, sort_tile=sort_tile)
+    b.step(2)
+    b.step(1)
+    for i, (s0, cfg, dt) in enumerate(sims):
+        s, a = b.read(i)
+        rs, ra = oracle.f32_evolve(s0, 3, dt, ocfg(oracle, cfg))
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+    st, hsh = b.force_evidence(1)
+    t = oracle.tree_from_state(b.read(1)[0], ocfg(oracle, sims[1][1]))
+    _, ostats, ohash = oracle.traverse_f32(*t.pack(), evidence=True)
+    assert np.array_equal(st, ostats) and np.array_equal(hsh, ohash)
+    b.close()
+
+
+def test_server_results_same_alone_and_in_mix(oracle, gpu):
+    sims = mixed_sims()
+    steps = [10, 30, 20, 10, 40]
+    print("server mix", flush=True)
+    srv = Server(gpu, hbm_budget_bytes=8 << 30, steps_per_round=10)
+    ids = [srv.submit(s, c, dt, k) for (s, c, dt), k in zip(sims, steps)]
+    done = srv.drain()
+    assert sorted(d.id for d in done) == sorted(ids)
+    mixed = {i: srv.take(i) for i in ids}
+    for i, ((s0, cfg, dt), k) in zip(ids, zip(sims, steps)):
+        rs, ra = oracle.f32_evolve(s0, k, dt, ocfg(oracle, cfg))
+        s, a = mixed[i]
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+    # the same request alone in a fresh server
+    alone = Server(gpu, hbm_budget_bytes=8 << 30, steps_per_round=10)
+    j = alone.submit(sims[1][0], sims[1][1], sims[1][2], steps[1])
+    alone.drain()
+    s, a = alone.take(j)
+    assert np.array_equal(s.view(np.uint32), mixed[ids[1]][0].view(np.uint32))
+
+
+def test_server_budget_admission_and_staggered_arrivals(oracle, gpu):
+    sims = mixed_sims()
+    one = Server.request_bytes(5000)
+    # budget holds the 5000-body request plus a little: requests queue behind it
+    srv = Server(gpu, hbm_budget_bytes=one + Server.request_bytes(2049), steps_per_round=10)
+    a = srv.submit(*sims[1], 30)
+    srv.round()
+    b = srv.submit(*sims[0], 10)      # arrives while `a` is in flight
+    c = srv.submit(*sims[3], 10)
+    q, infl, res = srv.stats()
+    assert infl == 1 and q == 2
+    done = srv.drain()
+    assert {d.id for d in done} == {a, b, c}
+    for rid, (s0, cfg, dt), k in [(a, sims[1], 30), (b, sims[0], 10), (c, sims[3], 10)]:
+        rs, _ = oracle.f32_evolve(s0, k, dt, ocfg(oracle, cfg))
+        assert np.array_equal(srv.take(rid)[0].view(np.uint32), rs.view(np.uint32))
+    with pytest.raises(Exception, match="HBM budget"):
+        Server(gpu, hbm_budget_bytes=1 << 20, steps_per_round=10).submit(*sims[1], 10)
