@@ -105,6 +105,30 @@ struct TileRef {
     uint32_t base;
 };
 
+// Read-only views in the constant address space: loads at wave-uniform
+// addresses become scalar (s_load) loads.
+typedef const __attribute__((address_space(4))) uint32_t* ScU32;
+
+__device__ __forceinline__ CellT sc_cell(ScU32 base, uint32_t i) {
+    const ScU32 w = base + 8u * i;
+    CellT c;
+    c.comx = __uint_as_float(w[0]);
+    c.comy = __uint_as_float(w[1]);
+    c.mass = __uint_as_float(w[2]);
+    c.T = __uint_as_float(w[3]);
+    c.start = w[4];
+    c.end = w[5];
+    c.skip = w[6];
+    c.info = w[7];
+    return c;
+}
+
+__device__ __forceinline__ float4 sc_f4(ScU32 base, uint32_t i) {
+    const ScU32 w = base + 4u * i;
+    return make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                       __uint_as_float(w[3]));
+}
+
 // ----------------------------------------------------------------------------
 // wavefront primitives (wave64, DPP within 16-lane rows + lane reads across rows)
 // ----------------------------------------------------------------------------
@@ -259,6 +283,37 @@ __device__ float accept_threshold(float half, float theta) {
     return __uint_as_float(hi);
 }
 
+// Correctly rounded f32 square root and reciprocal from the f64 hardware
+// approximations: the f32 result is the f64 approximation rounded once.  This is
+// exact when the f64 error is smaller than the distance from the true value to
+// the nearest f32 rounding midpoint, which k_check_rounding verifies
+// exhaustively for every positive f32 input against the compiler's correctly
+// rounded sqrtf and division.
+// Square root of a non-negative f32: one Goldschmidt step in f64 from the
+// hardware reciprocal square root (g ~ sqrt(x), h ~ 1/(2 sqrt(x))), rounded once
+// to f32.  Zero, +inf and NaN pass through as sqrtf returns them.
+__device__ __forceinline__ float sqrt_rn(float xf) {
+    const double x = (double)xf;
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    const double h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    const bool finite_pos = xf > 0.0f && xf < __builtin_inff();
+    return finite_pos ? (float)g : (xf < 0.0f ? __builtin_nanf("") : xf);
+}
+
+// Reciprocal of a non-negative f32: f64 hardware reciprocal plus one Newton
+// step, rounded once to f32.  1/0 = +inf and 1/inf = 0 as for the division.
+__device__ __forceinline__ float rcp_rn(float sf) {
+    const double s = (double)sf;
+    double y = __builtin_amdgcn_rcp(s);
+    const double e = __builtin_fma(-s, y, 1.0);
+    const double y1 = __builtin_fma(e, y, y);
+    const bool finite_pos = sf > 0.0f && sf < __builtin_inff();
+    return finite_pos ? (float)y1 : (float)y;
+}
+
 // point_mass (nbody.wgsl:35-41) with inverseSqrt = 1 / sqrt, each correctly rounded.
 __device__ __forceinline__ void point_mass(float tx, float ty, float sx, float sy, float m,
                                            float eps2, float G, float& ax, float& ay) {
@@ -267,14 +322,34 @@ __device__ __forceinline__ void point_mass(float tx, float ty, float sx, float s
     float dd = dx * dx;
     float ee = dy * dy;
     float r2 = (dd + ee) + eps2;
-    float s = __builtin_sqrtf(r2);
-    float inv_r = 1.0f / s;
+    float s = sqrt_rn(r2);
+    float inv_r = rcp_rn(s);
     float scale = G * m;
     scale = scale * inv_r;
     scale = scale * inv_r;
     scale = scale * inv_r;
     ax = ax + dx * scale;
     ay = ay + dy * scale;
+}
+
+// Exhaustive check of sqrt_rn / rcp_rn against the compiler's correctly rounded
+// sqrtf and division over every non-negative f32 bit pattern (+0 to +inf).
+// counts[0..1]: mismatches; first[0..1]: smallest mismatching bit pattern.
+__global__ void k_check_rounding(unsigned long long* __restrict__ counts,
+                                 uint32_t* __restrict__ first) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= 0x7f800000u; b += stride) {
+        const float x = __uint_as_float(b);
+        if (__float_as_uint(sqrt_rn(x)) != __float_as_uint(__builtin_sqrtf(x))) {
+            atomicAdd(&counts[0], 1ull);
+            atomicMin(&first[0], b);
+        }
+        if (__float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x)) {
+            atomicAdd(&counts[1], 1ull);
+            atomicMin(&first[1], b);
+        }
+        if (b > 0x7f800000u - stride) break;
+    }
 }
 
 __device__ __forceinline__ void mix64(unsigned long long& h, unsigned long long w) {
@@ -942,15 +1017,18 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
         const uint32_t n = S.n;
         const uint32_t p = tr.base + lane;
         const bool valid = p < n;
-        const CellT* __restrict__ cl = S.cells;
-        const float4* __restrict__ sb = S.sbody;
-        const uint32_t ncells = S.meta->ncells;
+        // Cells and sorted bodies are read at wave-uniform indices: scalar loads
+        // through the constant address space (read-only for this kernel).
+        const ScU32 cl = (ScU32)(uintptr_t)S.cells;
+        const ScU32 sbs = (ScU32)(uintptr_t)S.sbody;
+        const ScU32 vals = (ScU32)(uintptr_t)S.val0;
+        const uint32_t ncells = __builtin_amdgcn_readfirstlane(S.meta->ncells);
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
         float tx = 0.f, ty = 0.f;
         if (valid) {
-            const float4 b = sb[p];
+            const float4 b = S.sbody[p];
             tx = b.x;
             ty = b.y;
         }
@@ -958,9 +1036,12 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
         uint32_t nxt = valid ? 0u : 0xffffffffu;
         uint32_t visited = 0, accepted = 0, direct = 0;
         unsigned long long h = 0xcbf29ce484222325ull;
+        const uint32_t last = ncells > 0 ? ncells - 1 : 0;
         uint32_t c = 0;
+        CellT cell = sc_cell(cl, 0);
         while (c < ncells) {
-            const CellT cell = cl[c];
+            // prefetch the cell a descent would visit next
+            const CellT pre = sc_cell(cl, min(c + 1, last));
             const bool act = (nxt == c);
             // the cursor only moves forward (skip > c in a well-formed tree), so
             // the walk ends after at most ncells iterations
@@ -968,13 +1049,34 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
             if (cell.info & kInfoLeaf) {
                 if (act) {
                     const uint32_t qend = min(cell.end, n);
-                    for (uint32_t q = cell.start; q < qend; ++q) {
-                        if (q != p) {
-                            const float4 b = sb[q];
-                            point_mass(tx, ty, b.x, b.y, b.z, eps2, G, ax, ay);
-                            if (kEvidence) {
-                                ++direct;
-                                mix64(h, (1ull << 60) | (unsigned long long)S.val0[q]);
+                    for (uint32_t q = cell.start; q < qend; q += 4) {
+                        // four consecutive sorted bodies in one 64-byte scalar load; the
+                        // arena places other buffers after the bodies, so reading past
+                        // the last body stays inside the allocation
+                        uint32_t w[16];
+#pragma unroll
+                        for (uint32_t k = 0; k < 16; ++k) w[k] = sbs[4u * q + k];
+                        float4 bb[4];
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; ++j)
+                            bb[j] = make_float4(__uint_as_float(w[4 * j]), __uint_as_float(w[4 * j + 1]),
+                                                __uint_as_float(w[4 * j + 2]), 0.0f);
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; ++j) {
+                            const uint32_t qj = q + j;
+                            if (qj >= qend) break;  // wave-uniform
+                            const bool use = qj != p;
+                            // the target's own slot is computed and discarded (a select,
+                            // not a branch), so the loads above stay ahead of the arithmetic
+                            float px = ax, py = ay;
+                            point_mass(tx, ty, bb[j].x, bb[j].y, bb[j].z, eps2, G, px, py);
+                            ax = use ? px : ax;
+                            ay = use ? py : ay;
+                            if (use) {
+                                if (kEvidence) {
+                                    ++direct;
+                                    mix64(h, (1ull << 60) | (unsigned long long)vals[qj]);
+                                }
                             }
                         }
                     }
@@ -1001,8 +1103,14 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
                     nxt = c + 1;
                 }
             }
-            if (__ballot(nxt == c + 1)) c = c + 1;
-            else c = wave_min_u32(nxt);
+            if (__ballot(nxt == c + 1)) {
+                c = c + 1;
+                cell = pre;
+            } else {
+                c = __builtin_amdgcn_readfirstlane(wave_min_u32(nxt));
+                cell = sc_cell(cl, min(c, last));
+            }
+            c = __builtin_amdgcn_readfirstlane(c);
         }
         if (valid) {
             const uint32_t body = S.val0[p];

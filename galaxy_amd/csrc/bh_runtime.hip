@@ -1012,6 +1012,30 @@ int gbh_tree_read_flat(gbh_tree* t, GbhGpuEntry* entries, GbhGpuCell* cells, uin
 
 double gbh_tree_last_force_ms(gbh_tree* t) { return t ? t->last_force_ms : 0.0; }
 
+int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]) {
+    return guarded([&] {
+        if (!dev || !mismatches || !first_bad) {
+            set_error("null argument");
+            return -1;
+        }
+        hipStream_t s = dev->stream;
+        unsigned long long* counts = nullptr;
+        uint32_t* first = nullptr;
+        HIP_OK(hipMalloc((void**)&counts, 16));
+        HIP_OK(hipMalloc((void**)&first, 8));
+        HIP_OK(hipMemsetAsync(counts, 0, 16, s));
+        HIP_OK(hipMemsetAsync(first, 0xff, 8, s));
+        hipLaunchKernelGGL(k_check_rounding, dim3(4096), dim3(256), 0, s, counts, first);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(mismatches, counts, 16, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(first_bad, first, 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipFree(counts));
+        HIP_OK(hipFree(first));
+        return 0;
+    });
+}
+
 uint64_t gbh_tree_buffer_bytes(gbh_tree* t) { return t ? (uint64_t)t->bytes : 0; }
 
 uint64_t gbh_sparse_layout_bytes(uint32_t n) {

@@ -249,6 +249,11 @@ int gbh_server_take_result(gbh_server* s, uint64_t id, GbhEvolveState* out, GbhG
 /* queued, in-flight, resident bytes */
 int gbh_server_stats(gbh_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* resident);
 
+/* Exhaustive device check (no reference counterpart) that the traversal's f32
+ * square root and reciprocal are correctly rounded for every positive f32:
+ * mismatches[0] = sqrt, [1] = reciprocal; first_bad = smallest failing pattern. */
+int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]);
+
 /* ---------------------------------------------------------------------------
  * Seeded synthetic initial conditions (host side; no GPU needed).
  * The f32 records are the canonical state; generation is f64 then one rounding.
