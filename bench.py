@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--only", choices=["plain", "batched", "copy"], default=None,
+                    help="run one leg only (for profiling); prints that leg's result")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the simulations of this workload are independent and "
@@ -181,7 +183,26 @@ def main():
     from galaxy_amd import NbodyGpu
 
     gpu = NbodyGpu()
-    log(f"device: {gpu.info()['name']}")
+    log(f"device: {gpu.info()}")
+    if args.only == "copy":
+        import ctypes as C
+
+        from galaxy_amd import _lib as L
+
+        ms = C.c_double()
+        nbytes = 1 << 30
+        for _ in range(3):
+            if L.lib().gbh_copy_calibration(gpu._h, nbytes, C.byref(ms)) != 0:
+                raise RuntimeError(L.lib().gbh_last_error().decode())
+        print(json.dumps({"copy_bytes": nbytes, "copy_ms": ms.value,
+                          "copy_GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
+    if args.only == "plain":
+        print(json.dumps(plain_run(gpu, args.steps, args.warmup)), flush=True)
+        return
+    if args.only == "batched":
+        print(json.dumps(batched_run(gpu, args.steps, args.warmup)), flush=True)
+        return
     log("plain drop-in, cold collapse N=65536 ...")
     plain = plain_run(gpu, args.steps, args.warmup)
     log(f"  plain {plain['body_steps_per_s']:.3e} body-steps/s, {plain['ms_per_step']:.3f} ms/step")

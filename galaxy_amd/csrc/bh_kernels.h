@@ -53,6 +53,21 @@ struct __attribute__((aligned(32))) CellT {
     uint32_t start, end, skip, info;
 };
 
+// Node stream, 32 bytes per node: the preorder of the internal cells with every
+// leaf replaced by its bodies (leaves are never accepted whole, nbody.wgsl:71-84,
+// so a leaf contributes exactly its bodies' direct terms, in sorted order).
+// One rule serves both kinds: a target at sorted position p interacts with a
+// node iff p is outside [start, start + count) and d2 >= T, and then continues
+// at `skip`, else at the next node.  Body nodes have count 1 (self excluded),
+// T = 0 and skip = index + 1; internal nodes carry their cell's aggregates.
+struct __attribute__((aligned(32))) NodeT {
+    float x, y, m, T;
+    uint32_t start, count, skip, tag;  // tag: kTagBody | body index, or the depth
+};
+constexpr uint32_t kTagBody = 1u << 31;
+constexpr uint32_t kTagFirst = 1u << 30;  // first body of its leaf
+constexpr uint32_t kTagValue = kTagFirst - 1u;
+
 struct Agg {  // f64 aggregates of a cell (mass, centre of mass)
     double m, x, y, pad;
 };
@@ -62,6 +77,7 @@ struct SimMeta {
     float bounds_f32[4];    // TreeMeta.bounds
     float T[20];            // per-depth acceptance thresholds (17 used)
     uint32_t ncells;
+    uint32_t nnodes;        // internal cells + bodies
     uint32_t nspan;
     uint32_t err;
     uint32_t theta_bits;    // f32 theta the thresholds were computed for
@@ -96,6 +112,8 @@ struct __attribute__((aligned(16))) SimDesc {
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
     uint32_t* span;    // spanning internal cells (span_cap entries)
+    NodeT* nodes;      // traversal node stream (cell_cap + n entries)
+    uint32_t* nbase;   // per sorted position: node index of the column's first cell
     float4* bpart;     // bounds partials, one per kTile bodies
     SimMeta* meta;
 };
@@ -104,30 +122,6 @@ struct TileRef {
     uint32_t sim;
     uint32_t base;
 };
-
-// Read-only views in the constant address space: loads at wave-uniform
-// addresses become scalar (s_load) loads.
-typedef const __attribute__((address_space(4))) uint32_t* ScU32;
-
-__device__ __forceinline__ CellT sc_cell(ScU32 base, uint32_t i) {
-    const ScU32 w = base + 8u * i;
-    CellT c;
-    c.comx = __uint_as_float(w[0]);
-    c.comy = __uint_as_float(w[1]);
-    c.mass = __uint_as_float(w[2]);
-    c.T = __uint_as_float(w[3]);
-    c.start = w[4];
-    c.end = w[5];
-    c.skip = w[6];
-    c.info = w[7];
-    return c;
-}
-
-__device__ __forceinline__ float4 sc_f4(ScU32 base, uint32_t i) {
-    const ScU32 w = base + 4u * i;
-    return make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
-                       __uint_as_float(w[3]));
-}
 
 // ----------------------------------------------------------------------------
 // wavefront primitives (wave64, DPP within 16-lane rows + lane reads across rows)
@@ -141,19 +135,6 @@ __device__ __forceinline__ uint32_t dpp_shr_u32(uint32_t v, uint32_t identity, i
     case 4: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x114, 0xf, 0xf, false);
     default: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x118, 0xf, 0xf, false);
     }
-}
-
-// Minimum over all 64 lanes (every lane must be active).
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    v = min(v, dpp_shr_u32(v, 0xffffffffu, 1));
-    v = min(v, dpp_shr_u32(v, 0xffffffffu, 2));
-    v = min(v, dpp_shr_u32(v, 0xffffffffu, 4));
-    v = min(v, dpp_shr_u32(v, 0xffffffffu, 8));
-    uint32_t a = __builtin_amdgcn_readlane(v, 15);
-    uint32_t b = __builtin_amdgcn_readlane(v, 31);
-    uint32_t c = __builtin_amdgcn_readlane(v, 47);
-    uint32_t d = __builtin_amdgcn_readlane(v, 63);
-    return min(min(a, b), min(c, d));
 }
 
 // Inclusive prefix sum over the 64 lanes (every lane must be active).
@@ -229,6 +210,39 @@ __device__ uint32_t lookback(uint32_t* flags, uint32_t stride, uint32_t t, uint3
         if (f & kFlagInc) break;
     }
     return excl;
+}
+
+// The same look-back over 64-bit flags that carry two 31-bit counts (status in
+// bits 62-63): the topology scan counts cells and columns together.
+constexpr unsigned long long kFlag64Agg = 1ull << 62;
+constexpr unsigned long long kFlag64Inc = 2ull << 62;
+constexpr unsigned long long kFlag64Value = (1ull << 62) - 1ull;
+
+__device__ __forceinline__ unsigned long long pack2(uint32_t a, uint32_t b) {
+    return (unsigned long long)a | ((unsigned long long)b << 31);
+}
+
+__device__ unsigned long long lookback2(unsigned long long* flags, uint32_t t, uint32_t* err) {
+    uint32_t a = 0, b = 0;
+    uint32_t k = t;
+    while (k > 0) {
+        --k;
+        unsigned long long f = 0;
+        for (uint32_t spins = 0;; ++spins) {
+            f = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (f != 0) break;
+            if (spins >= kSpinLimit) {
+                atomicOr(err, kErrLookback);
+                f = kFlag64Inc;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        a += (uint32_t)(f & 0x7fffffffull);
+        b += (uint32_t)((f >> 31) & 0x7fffffffull);
+        if (f & kFlag64Inc) break;
+    }
+    return pack2(a, b);
 }
 
 // ----------------------------------------------------------------------------
@@ -314,14 +328,11 @@ __device__ __forceinline__ float rcp_rn(float sf) {
     return finite_pos ? (float)y1 : (float)y;
 }
 
-// point_mass (nbody.wgsl:35-41) with inverseSqrt = 1 / sqrt, each correctly rounded.
-__device__ __forceinline__ void point_mass(float tx, float ty, float sx, float sy, float m,
-                                           float eps2, float G, float& ax, float& ay) {
-    float dx = sx - tx;
-    float dy = sy - ty;
-    float dd = dx * dx;
-    float ee = dy * dy;
-    float r2 = (dd + ee) + eps2;
+// point_mass (nbody.wgsl:35-41) with inverseSqrt = 1 / sqrt, each correctly
+// rounded, from the offset (dx, dy) and d2 = dx*dx + dy*dy.
+__device__ __forceinline__ void point_mass_d(float dx, float dy, float d2, float m, float eps2,
+                                             float G, float& ax, float& ay) {
+    float r2 = d2 + eps2;
     float s = sqrt_rn(r2);
     float inv_r = rcp_rn(s);
     float scale = G * m;
@@ -330,6 +341,15 @@ __device__ __forceinline__ void point_mass(float tx, float ty, float sx, float s
     scale = scale * inv_r;
     ax = ax + dx * scale;
     ay = ay + dy * scale;
+}
+
+__device__ __forceinline__ void point_mass(float tx, float ty, float sx, float sy, float m,
+                                           float eps2, float G, float& ax, float& ay) {
+    const float dx = sx - tx;
+    const float dy = sy - ty;
+    const float dd = dx * dx;
+    const float ee = dy * dy;
+    point_mass_d(dx, dy, dd + ee, m, eps2, G, ax, ay);
 }
 
 // Exhaustive check of sqrt_rn / rcp_rn against the compiler's correctly rounded
@@ -366,7 +386,7 @@ __device__ __forceinline__ void mix64(unsigned long long& h, unsigned long long 
 __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ sims,
                                                   const TileRef* __restrict__ tiles,
                                                   uint32_t* __restrict__ ctl,
-                                                  uint32_t* __restrict__ tflags,
+                                                  unsigned long long* __restrict__ tflags,
                                                   uint32_t ntopo, int mode) {
     __shared__ float s_red[4][kTile / 64];
     const TileRef tr = tiles[blockIdx.x];
@@ -376,7 +396,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     if (blockIdx.x == 0 && threadIdx.x < 6) ctl[threadIdx.x] = 0u;  // tickets + queue
     // one look-back flag of the topology scan per block (there are fewer
     // topology tiles than bounds tiles)
-    if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0u;
+    if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0ull;
     if (tr.base == 0) {
         for (uint32_t k = threadIdx.x; k < 4 * 256; k += kTile) S.meta->hist[k] = 0u;
     }
@@ -717,10 +737,10 @@ __device__ __forceinline__ uint32_t group_end(const uint32_t* __restrict__ K, ui
 __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restrict__ sims,
                                                           const TileRef* __restrict__ ttiles,
                                                           uint32_t* __restrict__ ctl,
-                                                          uint32_t* __restrict__ tflags) {
+                                                          unsigned long long* __restrict__ tflags) {
     __shared__ uint32_t s_scan[kTopoTile / 64];
     __shared__ uint32_t s_ticket;
-    __shared__ uint32_t s_excl;
+    __shared__ uint32_t s_excl, s_excl_cols;
     if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
     __syncthreads();
     const uint32_t t = s_ticket;
@@ -751,26 +771,40 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
             }
         }
     }
+    // cells (low 16 bits) and columns (high 16 bits) of the tile in one scan:
+    // a tile holds at most 17 * 1024 cells
     uint32_t total;
-    const uint32_t mine = block_excl_sum<kTopoTile>(count, s_scan, total);
+    const uint32_t packed = count | ((count > 0 ? 1u : 0u) << 16);
+    const uint32_t mine = block_excl_sum<kTopoTile>(packed, s_scan, total);
     if (threadIdx.x == 0) {
-        uint32_t* f = tflags + t;
-        uint32_t excl;
+        unsigned long long* f = tflags + t;
+        const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
+        unsigned long long excl;
         if (t == S.first_topo_tile) {
             excl = 0;
-            flag_store(f, kFlagInc | total);
+            __hip_atomic_store(f, kFlag64Inc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            flag_store(f, kFlagAgg | total);
-            excl = lookback(tflags, 1, t, &ctl[kCtlError]);
-            flag_store(f, kFlagInc | (excl + total));
+            __hip_atomic_store(f, kFlag64Agg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            excl = lookback2(tflags, t, &ctl[kCtlError]);
+            __hip_atomic_store(f, kFlag64Inc | (excl + tot), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_excl = excl;
-        if (tr.base + kTopoTile >= n) S.meta->ncells = excl + total;
+        s_excl = (uint32_t)(excl & 0x7fffffffull);
+        s_excl_cols = (uint32_t)(excl >> 31);
+        if (tr.base + kTopoTile >= n) {
+            const uint32_t ncells = s_excl + (total & 0xffffu);
+            const uint32_t ncols = s_excl_cols + (total >> 16);
+            S.meta->ncells = ncells;
+            S.meta->nnodes = ncells - ncols + n;  // internal cells + bodies
+        }
     }
     __syncthreads();
     if (!valid) return;
-    const uint32_t base = s_excl + mine;
+    const uint32_t base = s_excl + (mine & 0xffffu);
+    // node index of the column's first cell: internal cells and bodies before p
+    const uint32_t nb = base - (s_excl_cols + (mine >> 16)) + p;
     S.cbase[p] = base;
+    S.nbase[p] = nb;
     S.cdmin[p] = (uint8_t)dmin;
     S.ccount[p] = (uint8_t)count;
     S.cparent[p] = pl;
@@ -803,12 +837,24 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
         uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
         if (leaf) {
             double mass = 0.0, wx = 0.0, wy = 0.0;
+            const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
             for (uint32_t q = p; q < e; ++q) {
                 const float4 b = S.sbody[q];
                 const double bm = b.z;
                 mass = mass + bm;
                 wx = wx + bm * (double)b.x;
                 wy = wy + bm * (double)b.y;
+                const uint32_t ni = nfirst + (q - p);
+                NodeT nd;
+                nd.x = b.x;
+                nd.y = b.y;
+                nd.m = b.z;
+                nd.T = 0.0f;
+                nd.start = q;
+                nd.count = 1u;
+                nd.skip = ni + 1u;
+                nd.tag = kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue);
+                S.nodes[ni] = nd;
             }
             const double comx = wx / mass, comy = wy / mass;
             S.cagg[idx] = Agg{mass, comx, comy, 0.0};
@@ -864,16 +910,25 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
     const uint32_t p = tr.base + threadIdx.x;
     const bool valid = p < n;
     const uint32_t ncells = S.meta->ncells;
-    uint32_t count = 0, dmin = 17, base = 0;
+    const uint32_t nnodes = S.meta->nnodes;
+    uint32_t count = 0, dmin = 17, base = 0, nb = 0;
     if (valid) {
         count = S.ccount[p];
         dmin = S.cdmin[p];
         base = S.cbase[p];
+        nb = S.nbase[p];
     }
     if (count > 0 && base + count <= S.cell_cap) {
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t e = S.cells[base + k].end;
-            S.cells[base + k].skip = e < n ? S.cbase[e] : ncells;
+            if (k + 1u < count) {  // internal: its node (aggregates follow below)
+                NodeT* nd = &S.nodes[nb + k];
+                nd->T = S.cells[base + k].T;
+                nd->start = p;
+                nd->count = e - p;
+                nd->skip = e < n ? S.nbase[e] : nnodes;
+                nd->tag = dmin + k;
+            }
         }
         if (dmin >= 1) {
             const uint32_t pl = S.cparent[p];
@@ -899,6 +954,10 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                 S.cells[idx].comx = (float)a.x;
                 S.cells[idx].comy = (float)a.y;
                 S.cells[idx].mass = (float)a.m;
+                NodeT* nd = &S.nodes[nb + ((uint32_t)d - dmin)];
+                nd->x = (float)a.x;
+                nd->y = (float)a.y;
+                nd->m = (float)a.m;
             } else {
                 const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
                 if (slot < S.span_cap) {
@@ -966,6 +1025,11 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             S.cells[idx].comx = (float)comx;
             S.cells[idx].comy = (float)comy;
             S.cells[idx].mass = (float)mass;
+            const uint32_t st = S.cells[idx].start;
+            NodeT* nd = &S.nodes[S.nbase[st] + (depth - S.cdmin[st])];
+            nd->x = (float)comx;
+            nd->y = (float)comy;
+            nd->m = (float)mass;
         }
         __syncthreads();
     }
@@ -989,23 +1053,37 @@ __global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
     const uint32_t nc = S.meta->ncells;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
         S.cells[i].T = S.meta->T[S.cells[i].info & 255u];
+    const uint32_t nn = S.meta->nnodes;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
+        const uint32_t tag = S.nodes[i].tag;
+        if (!(tag & kTagBody)) S.nodes[i].T = S.meta->T[tag & 255u];
+    }
 }
 
 // ----------------------------------------------------------------------------
 // K7: force traversal (bh_traverse, nbody.wgsl:43-123), stackless over the
-// preorder cells.  A wavefront owns 64 consecutive sorted targets and walks the
-// union of their DFS sequences with one wave-uniform cursor; each lane acts
-// only on the cells of its own sequence, so its interaction list and f32
-// summation order are exactly the per-target DFS of the reference.  Work is
-// pulled from a global queue of (simulation, 64-target tile) items.
+// node stream with an independent cursor per lane.  Work is pulled from a
+// global queue of (simulation, 64 consecutive sorted targets) items, larger
+// simulations first.  Each lane walks its own target's sequence one node per
+// iteration with the single interaction rule of NodeT, so every iteration is
+// the same straight-line code for all lanes (no per-kind branches) and the
+// interaction is evaluated for every lane that has one.  The sequence is the
+// reference's per-target DFS: internal cells in preorder, a leaf as its bodies
+// in sorted order, self skipped, so the interaction list and the f32
+// summation order are the reference's.
 // ----------------------------------------------------------------------------
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef uint32_t U4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) F4v* GlobalF4;
+typedef const __attribute__((address_space(1))) U4v* GlobalU4;
+
 template <bool kEvidence>
-__global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ sims,
-                                                  const TileRef* __restrict__ wtiles,
-                                                  uint32_t nwtiles, uint32_t* __restrict__ ctl,
-                                                  uint32_t* __restrict__ ev_stats,
-                                                  unsigned long long* __restrict__ ev_hash,
-                                                  uint32_t ev_sim) {
+__global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restrict__ sims,
+                                                        const TileRef* __restrict__ wtiles,
+                                                        uint32_t nwtiles, uint32_t* __restrict__ ctl,
+                                                        uint32_t* __restrict__ ev_stats,
+                                                        unsigned long long* __restrict__ ev_hash,
+                                                        uint32_t ev_sim) {
     const uint32_t lane = lane_id();
     for (;;) {
         uint32_t item = 0;
@@ -1017,12 +1095,9 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
         const uint32_t n = S.n;
         const uint32_t p = tr.base + lane;
         const bool valid = p < n;
-        // Cells and sorted bodies are read at wave-uniform indices: scalar loads
-        // through the constant address space (read-only for this kernel).
-        const ScU32 cl = (ScU32)(uintptr_t)S.cells;
-        const ScU32 sbs = (ScU32)(uintptr_t)S.sbody;
-        const ScU32 vals = (ScU32)(uintptr_t)S.val0;
-        const uint32_t ncells = __builtin_amdgcn_readfirstlane(S.meta->ncells);
+        const GlobalF4 nf = (GlobalF4)(const void*)S.nodes;
+        const GlobalU4 nu = (GlobalU4)(const void*)S.nodes;
+        const uint32_t nn = __builtin_amdgcn_readfirstlane(S.meta->nnodes);
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
@@ -1033,84 +1108,37 @@ __global__ __launch_bounds__(256) void k_traverse(const SimDesc* __restrict__ si
             ty = b.y;
         }
         float ax = 0.f, ay = 0.f;
-        uint32_t nxt = valid ? 0u : 0xffffffffu;
         uint32_t visited = 0, accepted = 0, direct = 0;
         unsigned long long h = 0xcbf29ce484222325ull;
-        const uint32_t last = ncells > 0 ? ncells - 1 : 0;
-        uint32_t c = 0;
-        CellT cell = sc_cell(cl, 0);
-        while (c < ncells) {
-            // prefetch the cell a descent would visit next
-            const CellT pre = sc_cell(cl, min(c + 1, last));
-            const bool act = (nxt == c);
-            // the cursor only moves forward (skip > c in a well-formed tree), so
-            // the walk ends after at most ncells iterations
-            const uint32_t skip = max(cell.skip, c + 1);
-            if (cell.info & kInfoLeaf) {
-                if (act) {
-                    const uint32_t qend = min(cell.end, n);
-                    for (uint32_t q = cell.start; q < qend; q += 4) {
-                        // four consecutive sorted bodies in one 64-byte scalar load; the
-                        // arena places other buffers after the bodies, so reading past
-                        // the last body stays inside the allocation
-                        uint32_t w[16];
-#pragma unroll
-                        for (uint32_t k = 0; k < 16; ++k) w[k] = sbs[4u * q + k];
-                        float4 bb[4];
-#pragma unroll
-                        for (uint32_t j = 0; j < 4; ++j)
-                            bb[j] = make_float4(__uint_as_float(w[4 * j]), __uint_as_float(w[4 * j + 1]),
-                                                __uint_as_float(w[4 * j + 2]), 0.0f);
-#pragma unroll
-                        for (uint32_t j = 0; j < 4; ++j) {
-                            const uint32_t qj = q + j;
-                            if (qj >= qend) break;  // wave-uniform
-                            const bool use = qj != p;
-                            // the target's own slot is computed and discarded (a select,
-                            // not a branch), so the loads above stay ahead of the arithmetic
-                            float px = ax, py = ay;
-                            point_mass(tx, ty, bb[j].x, bb[j].y, bb[j].z, eps2, G, px, py);
-                            ax = use ? px : ax;
-                            ay = use ? py : ay;
-                            if (use) {
-                                if (kEvidence) {
-                                    ++direct;
-                                    mix64(h, (1ull << 60) | (unsigned long long)vals[qj]);
-                                }
-                            }
-                        }
-                    }
-                    nxt = skip;
-                    if (kEvidence) ++visited;
-                }
-            } else if (act) {
-                const float dx = cell.comx - tx;
-                const float dy = cell.comy - ty;
-                const float dd = dx * dx;
-                const float ee = dy * dy;
-                const float d2 = dd + ee;
-                const bool contains = (p - cell.start) < (cell.end - cell.start);
-                if (kEvidence) ++visited;
-                if (!contains && d2 >= cell.T) {
-                    point_mass(tx, ty, cell.comx, cell.comy, cell.mass, eps2, G, ax, ay);
-                    nxt = skip;
-                    if (kEvidence) {
+        uint32_t c = valid ? 0u : nn;
+        while (c < nn) {
+            const F4v g = nf[2u * c];      // x, y, m, T
+            const U4v l = nu[2u * c + 1u];  // start, count, skip, tag
+            const float dx = g.x - tx;
+            const float dy = g.y - ty;
+            const float dd = dx * dx;
+            const float ee = dy * dy;
+            const float d2 = dd + ee;
+            const bool contains = (p - l.x) < l.y;
+            const bool acc = !contains && d2 >= g.w;
+            if (kEvidence) {
+                if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++visited;  // a cell (a leaf: its first body)
+            }
+            if (acc) {
+                point_mass_d(dx, dy, d2, g.z, eps2, G, ax, ay);
+                if (kEvidence) {
+                    unsigned long long word;
+                    if (l.w & kTagBody) {
+                        ++direct;
+                        word = (1ull << 60) | (unsigned long long)(l.w & kTagValue);
+                    } else {
                         ++accepted;
-                        mix64(h, (2ull << 60) | ((unsigned long long)(cell.info & 255u) << 32) |
-                                     cell.start);
+                        word = (2ull << 60) | ((unsigned long long)(l.w & 255u) << 32) | l.x;
                     }
-                } else {
-                    nxt = c + 1;
+                    mix64(h, word);
                 }
             }
-            if (__ballot(nxt == c + 1)) {
-                c = c + 1;
-                cell = pre;
-            } else {
-                c = __builtin_amdgcn_readfirstlane(wave_min_u32(nxt));
-                cell = sc_cell(cl, min(c, last));
-            }
-            c = __builtin_amdgcn_readfirstlane(c);
+            c = acc ? max(l.z, c + 1u) : c + 1u;  // strictly increasing: the walk ends
         }
         if (valid) {
             const uint32_t body = S.val0[p];
@@ -1162,6 +1190,14 @@ __global__ __launch_bounds__(kTile) void k_kick_drift(const SimDesc* __restrict_
     p.y = p.y + dt * hvy;
     S.pos[i] = p;
     S.vel[i] = make_float2(hvx, hvy);
+}
+
+// Streaming copy used to calibrate the HBM traffic counters.
+__global__ __launch_bounds__(256) void k_copy_calibration(const float4* __restrict__ src,
+                                                          float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // ABI <-> SoA conversion of the frozen records

@@ -104,6 +104,8 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
     d.span = (uint32_t*)take(4ull * scap);
+    d.nodes = (NodeT*)take(sizeof(NodeT) * (cap + n));
+    d.nbase = (uint32_t*)take(4ull * n);
     d.bpart = (float4*)take(16ull * div_up(n, kTile));
     d.meta = (SimMeta*)take(sizeof(SimMeta));
     d.cell_cap = (uint32_t)cap;
@@ -139,7 +141,7 @@ struct SimSet {
     TileRef *d_t256 = nullptr, *d_tsort = nullptr, *d_ttopo = nullptr, *d_twave = nullptr;
     uint32_t* d_ctl = nullptr;
     uint32_t* d_sflags = nullptr;
-    uint32_t* d_tflags = nullptr;
+    unsigned long long* d_tflags = nullptr;
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
     size_t cap_sflags = 0, cap_tflags = 0;
     uint32_t sort_kpt = 8;
@@ -153,7 +155,8 @@ struct SimSet {
             if (p) (void)hipFree(p);
         d_desc = nullptr;
         d_t256 = d_tsort = d_ttopo = d_twave = nullptr;
-        d_ctl = d_sflags = d_tflags = nullptr;
+        d_ctl = d_sflags = nullptr;
+        d_tflags = nullptr;
         cap_desc = cap_t256 = cap_tsort = cap_ttopo = cap_twave = cap_sflags = cap_tflags = 0;
     }
 
@@ -217,7 +220,7 @@ struct SimSet {
         put(d_twave, twave);
         HIP_OK(hipMemsetAsync(d_ctl, 0, sizeof(uint32_t) * kCtlWords, stream));
         HIP_OK(hipMemsetAsync(d_sflags, 0, sizeof(uint32_t) * 4ull * kRadix * std::max<size_t>(tsort.size(), 1), stream));
-        HIP_OK(hipMemsetAsync(d_tflags, 0, sizeof(uint32_t) * std::max<size_t>(ttopo.size(), 1), stream));
+        HIP_OK(hipMemsetAsync(d_tflags, 0, sizeof(unsigned long long) * std::max<size_t>(ttopo.size(), 1), stream));
         // persistent traversal grid: enough 256-thread blocks to fill the device
         uint32_t waves = (uint32_t)twave.size();
         uint32_t want = std::max<uint32_t>(1, (uint32_t)dev->cu_count * 8);
@@ -282,10 +285,10 @@ struct SimSet {
             HIP_OK(hipGetLastError());
         }
         if (ev_stats || ev_hash)
-            hipLaunchKernelGGL(k_traverse<true>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
+            hipLaunchKernelGGL(k_traverse_nodes<true>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
                                d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, ev_sim);
         else
-            hipLaunchKernelGGL(k_traverse<false>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
+            hipLaunchKernelGGL(k_traverse_nodes<false>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
                                d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, 0xffffffffu);
         HIP_OK(hipGetLastError());
     }
@@ -1011,6 +1014,35 @@ int gbh_tree_read_flat(gbh_tree* t, GbhGpuEntry* entries, GbhGpuCell* cells, uin
 }
 
 double gbh_tree_last_force_ms(gbh_tree* t) { return t ? t->last_force_ms : 0.0; }
+
+int gbh_copy_calibration(gbh_device* dev, uint64_t bytes, double* ms) {
+    return guarded([&] {
+        if (!dev || !ms || bytes < 16) {
+            set_error("copy calibration needs a device, an output and >= 16 bytes");
+            return -1;
+        }
+        hipStream_t s = dev->stream;
+        const size_t n = bytes / 16;
+        float4 *src = nullptr, *dst = nullptr;
+        HIP_OK(hipMalloc((void**)&src, n * 16));
+        HIP_OK(hipMalloc((void**)&dst, n * 16));
+        HIP_OK(hipMemsetAsync(src, 0, n * 16, s));
+        hipEvent_t a, b;
+        HIP_OK(hipEventCreate(&a));
+        HIP_OK(hipEventCreate(&b));
+        HIP_OK(hipEventRecord(a, s));
+        hipLaunchKernelGGL(k_copy_calibration, dim3(8192), dim3(256), 0, s, src, dst, n);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(b, s));
+        HIP_OK(hipStreamSynchronize(s));
+        *ms = ms_between(a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        HIP_OK(hipFree(src));
+        HIP_OK(hipFree(dst));
+        return 0;
+    });
+}
 
 int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]) {
     return guarded([&] {

@@ -254,6 +254,11 @@ int gbh_server_stats(gbh_server* s, uint32_t* queued, uint32_t* inflight, uint64
  * mismatches[0] = sqrt, [1] = reciprocal; first_bad = smallest failing pattern. */
 int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]);
 
+/* Device-to-device copy of `bytes` bytes by a plain streaming kernel
+ * (k_copy_calibration); GPU time in *ms.  Used to calibrate hardware traffic
+ * counters (no reference counterpart). */
+int gbh_copy_calibration(gbh_device* dev, uint64_t bytes, double* ms);
+
 /* ---------------------------------------------------------------------------
  * Seeded synthetic initial conditions (host side; no GPU needed).
  * The f32 records are the canonical state; generation is f64 then one rounding.
