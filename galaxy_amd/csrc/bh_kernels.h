@@ -189,58 +189,81 @@ __device__ __forceinline__ uint32_t flag_load(uint32_t* p) {
 
 // Exclusive prefix of tile `t` over earlier tiles of the same segment, whose
 // flags are at flags[k * stride].  The segment's first tile publishes an
-// inclusive value without looking back, so every wait ends.
+// inclusive value without looking back, so every wait ends.  Eight
+// predecessor flags are read per round (independent loads), so a walk over m
+// aggregate-only predecessors costs about m/8 memory latencies.
+constexpr uint32_t kLookBatch = 8;
+
 __device__ uint32_t lookback(uint32_t* flags, uint32_t stride, uint32_t t, uint32_t* err) {
     uint32_t excl = 0;
-    uint32_t k = t;
+    uint32_t k = t;  // flags [0, k) not yet consumed
+    uint32_t spins = 0;
     while (k > 0) {
-        --k;
-        uint32_t f = 0;
-        for (uint32_t spins = 0;; ++spins) {
-            f = flag_load(flags + (size_t)k * stride);
-            if (f != 0) break;
-            if (spins >= kSpinLimit) {
+        uint32_t f[kLookBatch];
+#pragma unroll
+        for (uint32_t j = 0; j < kLookBatch; ++j)
+            f[j] = flag_load(flags + (size_t)(j < k ? k - 1u - j : k - 1u) * stride);
+        uint32_t used = 0;
+        bool done = false;
+#pragma unroll
+        for (uint32_t j = 0; j < kLookBatch; ++j) {
+            if (done || used != j || j >= k || f[j] == 0u) continue;
+            excl += f[j] & kFlagValue;
+            used = j + 1u;
+            if (f[j] & kFlagInc) done = true;
+        }
+        if (done) return excl;
+        k -= used;
+        if (used == 0u) {  // nearest predecessor not published yet
+            if (++spins >= kSpinLimit) {
                 atomicOr(err, kErrLookback);
-                f = kFlagInc;
-                break;
+                return excl;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        excl += f & kFlagValue;
-        if (f & kFlagInc) break;
     }
     return excl;
 }
 
 // The same look-back over 64-bit flags that carry two 31-bit counts (status in
-// bits 62-63): the topology scan counts cells and columns together.
+// bits 62-63), done by a whole wavefront: lane l reads the flag of tile
+// k - 1 - l, so 64 predecessors cost one memory latency.  Every lane of the
+// calling wavefront must be active; all lanes return the prefix.
 constexpr unsigned long long kFlag64Agg = 1ull << 62;
 constexpr unsigned long long kFlag64Inc = 2ull << 62;
-constexpr unsigned long long kFlag64Value = (1ull << 62) - 1ull;
 
 __device__ __forceinline__ unsigned long long pack2(uint32_t a, uint32_t b) {
     return (unsigned long long)a | ((unsigned long long)b << 31);
 }
 
-__device__ unsigned long long lookback2(unsigned long long* flags, uint32_t t, uint32_t* err) {
+__device__ unsigned long long lookback2_wave(unsigned long long* flags, uint32_t t, uint32_t* err) {
+    const uint32_t lane = lane_id();
     uint32_t a = 0, b = 0;
     uint32_t k = t;
+    uint32_t spins = 0;
     while (k > 0) {
-        --k;
-        unsigned long long f = 0;
-        for (uint32_t spins = 0;; ++spins) {
-            f = __hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (f != 0) break;
-            if (spins >= kSpinLimit) {
-                atomicOr(err, kErrLookback);
-                f = kFlag64Inc;
-                break;
+        unsigned long long f = kFlag64Inc;  // lanes past tile 0 read as an inclusive zero
+        if (lane < k)
+            f = __hip_atomic_load(flags + (k - 1u - lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long inc = __ballot((f & kFlag64Inc) != 0ull);
+        const unsigned long long pending = __ballot(f == 0ull);
+        const uint32_t first_inc = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;
+        const unsigned long long need = first_inc >= 63u ? ~0ull : ((2ull << first_inc) - 1ull);
+        if (pending & need) {
+            if (++spins >= kSpinLimit) {
+                if (lane == 0) atomicOr(err, kErrLookback);
+                return pack2(a, b);
             }
             __builtin_amdgcn_s_sleep(1);
+            continue;
         }
-        a += (uint32_t)(f & 0x7fffffffull);
-        b += (uint32_t)((f >> 31) & 0x7fffffffull);
-        if (f & kFlag64Inc) break;
+        const bool take = lane <= first_inc;
+        const uint32_t va = wave_incl_sum_u32(take ? (uint32_t)(f & 0x7fffffffull) : 0u);
+        const uint32_t vb = wave_incl_sum_u32(take ? (uint32_t)((f >> 31) & 0x7fffffffull) : 0u);
+        a += __builtin_amdgcn_readlane(va, 63);
+        b += __builtin_amdgcn_readlane(vb, 63);
+        if (first_inc < 64u) return pack2(a, b);
+        k -= 64u;
     }
     return pack2(a, b);
 }
@@ -776,22 +799,25 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
     uint32_t total;
     const uint32_t packed = count | ((count > 0 ? 1u : 0u) << 16);
     const uint32_t mine = block_excl_sum<kTopoTile>(packed, s_scan, total);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
         unsigned long long* f = tflags + t;
         const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
         unsigned long long excl;
         if (t == S.first_topo_tile) {
             excl = 0;
-            __hip_atomic_store(f, kFlag64Inc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(f, kFlag64Inc | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(f, kFlag64Agg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            excl = lookback2(tflags, t, &ctl[kCtlError]);
-            __hip_atomic_store(f, kFlag64Inc | (excl + tot), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(f, kFlag64Agg | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            excl = lookback2_wave(tflags, t, &ctl[kCtlError]);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(f, kFlag64Inc | (excl + tot), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         }
         s_excl = (uint32_t)(excl & 0x7fffffffull);
         s_excl_cols = (uint32_t)(excl >> 31);
-        if (tr.base + kTopoTile >= n) {
+        if (threadIdx.x == 0 && tr.base + kTopoTile >= n) {
             const uint32_t ncells = s_excl + (total & 0xffffu);
             const uint32_t ncols = s_excl_cols + (total >> 16);
             S.meta->ncells = ncells;
