@@ -35,6 +35,16 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per traversal launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by the copy-kernel calibration), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "traverse_batched256_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def interactions(rt_or_batch, *args):
     st, _ = rt_or_batch(*args)
     return int(st[:, 1].sum() + st[:, 2].sum())
@@ -220,6 +230,7 @@ def main():
         cpu = cpu_baseline()
 
     trav_ms = batched["phase_ms"]["traversal"]
+    traffic = measured_traffic()
     achieved = FLOPS_PER_INTERACTION * batched["interactions"] / (trav_ms * 1e-3) / 1e12
     plain_trav = FLOPS_PER_INTERACTION * plain["interactions"] / (plain["traversal_ms"] * 1e-3) / 1e12
     line = {
@@ -250,13 +261,16 @@ def main():
         "phase_ms": batched["phase_ms"],
         "roofline": {
             "bound": "mfma",
-            "compute": "fp32-valu (no MFMA: traversal is not a dense contraction)",
-            "kernel": "k_traverse",
+            "compute": "fp32 vector ALU (no MFMA: the traversal is not a dense contraction)",
+            "kernel": "k_traverse_nodes",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic["traffic_bytes"] if traffic else None,
+            "traffic_source": "profiles/traverse_batched256_traffic.json (rocprofv3 FETCH_SIZE x%.3f + "
+                              "WRITE_SIZE x%.3f per launch)" % (traffic["fetch_scale"], traffic["write_scale"])
+                              if traffic else None,
             "interactions_per_launch": batched["interactions"],
             "kernel_ms": trav_ms,
             "plain_traversal_tflops": plain_trav,
