@@ -53,17 +53,32 @@ struct __attribute__((aligned(32))) CellT {
     uint32_t start, end, skip, info;
 };
 
-// Node stream, 32 bytes per node: the preorder of the internal cells with every
-// leaf replaced by its bodies (leaves are never accepted whole, nbody.wgsl:71-84,
-// so a leaf contributes exactly its bodies' direct terms, in sorted order).
-// One rule serves both kinds: a target at sorted position p interacts with a
-// node iff p is outside [start, start + count) and d2 >= T, and then continues
-// at `skip`, else at the next node.  Body nodes have count 1 (self excluded),
-// T = 0 and skip = index + 1; internal nodes carry their cell's aggregates.
-struct __attribute__((aligned(32))) NodeT {
-    float x, y, m, T;
-    uint32_t start, count, skip, tag;  // tag: kTagBody | body index, or the depth
+// Node stream: the preorder of the internal cells with every leaf replaced by
+// its bodies (leaves are never accepted whole, nbody.wgsl:71-84, so a leaf
+// contributes exactly its bodies' direct terms, in sorted order).  One rule
+// serves both kinds of node: a target interacts with node c iff its own body
+// node lies outside c's subtree [c, c + size) and d2 >= T, and then continues at
+// c + size, else at c + 1.  A body node's subtree is itself (size 1, T = 0), so
+// it interacts with every target but its own body.
+//
+// The traversal reads only the 16-byte hot record; T comes from a per-depth
+// table of the simulation (code 0..16 = depth, kCodeBody = the body marker with
+// T = 0).  The cold record carries the sorted range, the node's own threshold
+// and the evidence tag, for the evidence traversal and for host-built trees
+// whose ranges or thresholds do not follow the preorder (general mode).
+struct __attribute__((aligned(16))) NodeHot {
+    float x, y, m;
+    uint32_t w;  // (subtree size << kSizeShift) | code
 };
+struct __attribute__((aligned(16))) NodeCold {
+    uint32_t start, count;
+    float T;
+    uint32_t tag;  // kTagBody | body index, or the depth
+};
+constexpr uint32_t kSizeShift = 5;
+constexpr uint32_t kCodeMask = (1u << kSizeShift) - 1u;
+constexpr uint32_t kCodeBody = kCodeMask;           // T = 0
+constexpr uint32_t kMaxNodes = 1u << (32 - kSizeShift);  // subtree sizes fit the hot word
 constexpr uint32_t kTagBody = 1u << 31;
 constexpr uint32_t kTagFirst = 1u << 30;  // first body of its leaf
 constexpr uint32_t kTagValue = kTagFirst - 1u;
@@ -75,7 +90,9 @@ struct Agg {  // f64 aggregates of a cell (mass, centre of mass)
 struct SimMeta {
     double root[4];         // f64 root (cx, cy, half) of root_bounds
     float bounds_f32[4];    // TreeMeta.bounds
-    float T[20];            // per-depth acceptance thresholds (17 used)
+    float T[32];            // node-code thresholds: depth 0..16, kCodeBody -> 0
+    uint32_t mode;          // 0: preorder-consistent nodes and T table; 1: general (cold records)
+    uint32_t geom_ok;       // every position lies in a box of half-width < 1e18
     uint32_t ncells;
     uint32_t nnodes;        // internal cells + bodies
     uint32_t nspan;
@@ -112,8 +129,10 @@ struct __attribute__((aligned(16))) SimDesc {
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
     uint32_t* span;    // spanning internal cells (span_cap entries)
-    NodeT* nodes;      // traversal node stream (cell_cap + n entries)
+    NodeHot* nodes;    // traversal node stream (cell_cap + n entries)
+    NodeCold* ncold;   // its cold records
     uint32_t* nbase;   // per sorted position: node index of the column's first cell
+    uint32_t* bnode;   // per sorted position: node index of that body
     float4* bpart;     // bounds partials, one per kTile bodies
     SimMeta* meta;
 };
@@ -326,29 +345,42 @@ __device__ float accept_threshold(float half, float theta) {
 // the nearest f32 rounding midpoint, which k_check_rounding verifies
 // exhaustively for every positive f32 input against the compiler's correctly
 // rounded sqrtf and division.
-// Square root of a non-negative f32: one Goldschmidt step in f64 from the
+// Square root of a finite positive f32: one Goldschmidt step in f64 from the
 // hardware reciprocal square root (g ~ sqrt(x), h ~ 1/(2 sqrt(x))), rounded once
-// to f32.  Zero, +inf and NaN pass through as sqrtf returns them.
-__device__ __forceinline__ float sqrt_rn(float xf) {
+// to f32.
+__device__ __forceinline__ float sqrt_rn_pos(float xf) {
     const double x = (double)xf;
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
     const double h = 0.5 * y;
     const double r = __builtin_fma(-g, h, 0.5);
     g = __builtin_fma(g, r, g);
-    const bool finite_pos = xf > 0.0f && xf < __builtin_inff();
-    return finite_pos ? (float)g : (xf < 0.0f ? __builtin_nanf("") : xf);
+    return (float)g;
 }
 
-// Reciprocal of a non-negative f32: f64 hardware reciprocal plus one Newton
-// step, rounded once to f32.  1/0 = +inf and 1/inf = 0 as for the division.
-__device__ __forceinline__ float rcp_rn(float sf) {
+// Square root of a non-negative f32; zero, +inf and NaN pass through as sqrtf
+// returns them.
+__device__ __forceinline__ float sqrt_rn(float xf) {
+    const float g = sqrt_rn_pos(xf);
+    const bool finite_pos = xf > 0.0f && xf < __builtin_inff();
+    return finite_pos ? g : (xf < 0.0f ? __builtin_nanf("") : xf);
+}
+
+// Reciprocal of a finite positive f32: f64 hardware reciprocal plus one Newton
+// step, rounded once to f32.
+__device__ __forceinline__ double rcp_f64_step(float sf) {
     const double s = (double)sf;
-    double y = __builtin_amdgcn_rcp(s);
+    const double y = __builtin_amdgcn_rcp(s);
     const double e = __builtin_fma(-s, y, 1.0);
-    const double y1 = __builtin_fma(e, y, y);
+    return __builtin_fma(e, y, y);
+}
+
+__device__ __forceinline__ float rcp_rn_pos(float sf) { return (float)rcp_f64_step(sf); }
+
+// Reciprocal of a non-negative f32; 1/0 = +inf and 1/inf = 0 as for the division.
+__device__ __forceinline__ float rcp_rn(float sf) {
     const bool finite_pos = sf > 0.0f && sf < __builtin_inff();
-    return finite_pos ? (float)y1 : (float)y;
+    return finite_pos ? rcp_rn_pos(sf) : (float)__builtin_amdgcn_rcp((double)sf);
 }
 
 // point_mass (nbody.wgsl:35-41) with inverseSqrt = 1 / sqrt, each correctly
@@ -358,6 +390,21 @@ __device__ __forceinline__ void point_mass_d(float dx, float dy, float d2, float
     float r2 = d2 + eps2;
     float s = sqrt_rn(r2);
     float inv_r = rcp_rn(s);
+    float scale = G * m;
+    scale = scale * inv_r;
+    scale = scale * inv_r;
+    scale = scale * inv_r;
+    ax = ax + dx * scale;
+    ay = ay + dy * scale;
+}
+
+// The same where r2 = d2 + eps2 is known finite and positive (then sqrt_rn and
+// rcp_rn take their finite_pos branch, so the result is identical).
+__device__ __forceinline__ void point_mass_pos(float dx, float dy, float d2, float m, float eps2,
+                                               float G, float& ax, float& ay) {
+    float r2 = d2 + eps2;
+    float s = sqrt_rn_pos(r2);
+    float inv_r = rcp_rn_pos(s);
     float scale = G * m;
     scale = scale * inv_r;
     scale = scale * inv_r;
@@ -555,12 +602,18 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
             M->bounds_f32[3] = 0.0f;
             M->nspan = 0u;
             M->theta_bits = __float_as_uint(S.theta);
+            M->mode = 0u;
+            // |dx|, |dy| <= 2 * half between any two positions (bodies and
+            // centres of mass lie in the root box), so d2 stays finite
+            M->geom_ok = half < 1.0e18 ? 1u : 0u;
         }
         if (threadIdx.x < kLevels) {
             // half at depth d is half * 2^-d exactly in f64; one rounding to f32
             double hd = half;
             for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
             M->T[threadIdx.x] = accept_threshold((float)hd, S.theta);
+        } else if (threadIdx.x < 32) {
+            M->T[threadIdx.x] = threadIdx.x == kCodeBody ? 0.0f : __uint_as_float(0x7fc00000u);
         }
     }
     // Morton keys of this tile
@@ -871,16 +924,10 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
                 wx = wx + bm * (double)b.x;
                 wy = wy + bm * (double)b.y;
                 const uint32_t ni = nfirst + (q - p);
-                NodeT nd;
-                nd.x = b.x;
-                nd.y = b.y;
-                nd.m = b.z;
-                nd.T = 0.0f;
-                nd.start = q;
-                nd.count = 1u;
-                nd.skip = ni + 1u;
-                nd.tag = kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue);
-                S.nodes[ni] = nd;
+                S.nodes[ni] = NodeHot{b.x, b.y, b.z, (1u << kSizeShift) | kCodeBody};
+                S.ncold[ni] = NodeCold{q, 1u, 0.0f,
+                                       kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue)};
+                S.bnode[q] = ni;
             }
             const double comx = wx / mass, comy = wy / mass;
             S.cagg[idx] = Agg{mass, comx, comy, 0.0};
@@ -948,12 +995,10 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t e = S.cells[base + k].end;
             if (k + 1u < count) {  // internal: its node (aggregates follow below)
-                NodeT* nd = &S.nodes[nb + k];
-                nd->T = S.cells[base + k].T;
-                nd->start = p;
-                nd->count = e - p;
-                nd->skip = e < n ? S.nbase[e] : nnodes;
-                nd->tag = dmin + k;
+                const uint32_t ni = nb + k;
+                const uint32_t skip = e < n ? S.nbase[e] : nnodes;
+                S.nodes[ni].w = ((skip - ni) << kSizeShift) | (dmin + k);
+                S.ncold[ni] = NodeCold{p, e - p, S.meta->T[dmin + k], dmin + k};
             }
         }
         if (dmin >= 1) {
@@ -980,7 +1025,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                 S.cells[idx].comx = (float)a.x;
                 S.cells[idx].comy = (float)a.y;
                 S.cells[idx].mass = (float)a.m;
-                NodeT* nd = &S.nodes[nb + ((uint32_t)d - dmin)];
+                NodeHot* nd = &S.nodes[nb + ((uint32_t)d - dmin)];
                 nd->x = (float)a.x;
                 nd->y = (float)a.y;
                 nd->m = (float)a.m;
@@ -1052,7 +1097,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             S.cells[idx].comy = (float)comy;
             S.cells[idx].mass = (float)mass;
             const uint32_t st = S.cells[idx].start;
-            NodeT* nd = &S.nodes[S.nbase[st] + (depth - S.cdmin[st])];
+            NodeHot* nd = &S.nodes[S.nbase[st] + (depth - S.cdmin[st])];
             nd->x = (float)comx;
             nd->y = (float)comy;
             nd->m = (float)mass;
@@ -1081,8 +1126,8 @@ __global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
         S.cells[i].T = S.meta->T[S.cells[i].info & 255u];
     const uint32_t nn = S.meta->nnodes;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const uint32_t tag = S.nodes[i].tag;
-        if (!(tag & kTagBody)) S.nodes[i].T = S.meta->T[tag & 255u];
+        const uint32_t tag = S.ncold[i].tag;
+        if (!(tag & kTagBody)) S.ncold[i].T = S.meta->T[tag & 255u];
     }
 }
 
@@ -1091,17 +1136,83 @@ __global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
 // node stream with an independent cursor per lane.  Work is pulled from a
 // global queue of (simulation, 64 consecutive sorted targets) items, larger
 // simulations first.  Each lane walks its own target's sequence one node per
-// iteration with the single interaction rule of NodeT, so every iteration is
-// the same straight-line code for all lanes (no per-kind branches) and the
-// interaction is evaluated for every lane that has one.  The sequence is the
-// reference's per-target DFS: internal cells in preorder, a leaf as its bodies
-// in sorted order, self skipped, so the interaction list and the f32
-// summation order are the reference's.
+// iteration with the single interaction rule of the node stream, so every
+// iteration is the same straight-line code for all lanes (no per-kind
+// branches) and the interaction is evaluated for every lane that has one.  The
+// sequence is the reference's per-target DFS: internal cells in preorder, a
+// leaf as its bodies in sorted order, self skipped, so the interaction list and
+// the f32 summation order are the reference's.
+//
+// Per iteration a lane gathers one 16-byte hot node; the opening threshold of
+// its depth comes from the wavefront's copy of the simulation's table in LDS,
+// and containment is a comparison of node indices (the target's own body node
+// against the subtree [c, c + size)).  General-mode trees (host-built trees whose
+// cell ranges or thresholds do not follow the node stream) read the cold record
+// for the range and the threshold instead, exactly as bh_traverse tests them.
 // ----------------------------------------------------------------------------
 typedef float F4v __attribute__((ext_vector_type(4)));
 typedef uint32_t U4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) F4v* GlobalF4;
 typedef const __attribute__((address_space(1))) U4v* GlobalU4;
+
+struct WalkEvidence {
+    uint32_t visited = 0, accepted = 0, direct = 0;
+    unsigned long long h = 0xcbf29ce484222325ull;
+};
+
+// kFast: every r2 = d2 + eps2 of the walk is finite and positive (the
+// simulation's positions lie in a box of half-width < 1e18 and
+// 0 < eps2 <= FLT_MAX), where point_mass_pos equals point_mass_d bit for bit.
+template <bool kEvidence, bool kGeneral, bool kFast>
+__device__ __forceinline__ void walk_nodes(const NodeHot* __restrict__ nodes,
+                                           const NodeCold* __restrict__ ncold,
+                                           const float* __restrict__ tab, uint32_t c, uint32_t nn,
+                                           uint32_t p, uint32_t self, float tx, float ty, float eps2,
+                                           float G, float& ax, float& ay, WalkEvidence& ev) {
+    const GlobalF4 nh = (GlobalF4)(const void*)nodes;
+    const GlobalU4 nc = (GlobalU4)(const void*)ncold;
+    while (c < nn) {
+        const F4v g = nh[c];  // x, y, m, (size << kSizeShift) | code
+        const uint32_t w = __float_as_uint(g.w);
+        const float dx = g.x - tx;
+        const float dy = g.y - ty;
+        const float dd = dx * dx;
+        const float ee = dy * dy;
+        const float d2 = dd + ee;
+        const uint32_t size = w >> kSizeShift;
+        bool inside;
+        float T;
+        U4v l;
+        if (kGeneral || kEvidence) l = nc[c];  // start, count, T, tag
+        if (kGeneral) {
+            inside = (p - l.x) < l.y;
+            T = __uint_as_float(l.z);
+        } else {
+            inside = (self - c) < size;
+            T = tab[w & kCodeMask];
+        }
+        const bool acc = !inside && d2 >= T;
+        if (kEvidence) {
+            if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++ev.visited;  // a cell (a leaf: its first body)
+        }
+        if (acc) {
+            if (kFast) point_mass_pos(dx, dy, d2, g.z, eps2, G, ax, ay);
+            else point_mass_d(dx, dy, d2, g.z, eps2, G, ax, ay);
+            if (kEvidence) {
+                unsigned long long word;
+                if (l.w & kTagBody) {
+                    ++ev.direct;
+                    word = (1ull << 60) | (unsigned long long)(l.w & kTagValue);
+                } else {
+                    ++ev.accepted;
+                    word = (2ull << 60) | ((unsigned long long)(l.w & 255u) << 32) | l.x;
+                }
+                mix64(ev.h, word);
+            }
+        }
+        c += acc ? size : 1u;  // size >= 1: the walk ends
+    }
+}
 
 template <bool kEvidence>
 __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restrict__ sims,
@@ -1110,6 +1221,8 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
                                                         uint32_t* __restrict__ ev_stats,
                                                         unsigned long long* __restrict__ ev_hash,
                                                         uint32_t ev_sim) {
+    __shared__ float s_T[256 / 64][32];
+    float* tab = s_T[threadIdx.x >> 6];
     const uint32_t lane = lane_id();
     for (;;) {
         uint32_t item = 0;
@@ -1121,61 +1234,47 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const uint32_t n = S.n;
         const uint32_t p = tr.base + lane;
         const bool valid = p < n;
-        const GlobalF4 nf = (GlobalF4)(const void*)S.nodes;
-        const GlobalU4 nu = (GlobalU4)(const void*)S.nodes;
         const uint32_t nn = __builtin_amdgcn_readfirstlane(S.meta->nnodes);
+        const uint32_t general = __builtin_amdgcn_readfirstlane(S.meta->mode);
+        const uint32_t geom_ok = __builtin_amdgcn_readfirstlane(S.meta->geom_ok);
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
+        const bool fast = geom_ok != 0u && eps2 > 0.0f && eps2 <= 3.4028234663852886e38f;
+        // this wavefront's copy of the threshold table (LDS is in order per wavefront)
+        if (lane < 32) tab[lane] = S.meta->T[lane];
+        __builtin_amdgcn_wave_barrier();
         float tx = 0.f, ty = 0.f;
+        uint32_t self = 0;
         if (valid) {
             const float4 b = S.sbody[p];
             tx = b.x;
             ty = b.y;
+            if (!general) self = S.bnode[p];
         }
         float ax = 0.f, ay = 0.f;
-        uint32_t visited = 0, accepted = 0, direct = 0;
-        unsigned long long h = 0xcbf29ce484222325ull;
-        uint32_t c = valid ? 0u : nn;
-        while (c < nn) {
-            const F4v g = nf[2u * c];      // x, y, m, T
-            const U4v l = nu[2u * c + 1u];  // start, count, skip, tag
-            const float dx = g.x - tx;
-            const float dy = g.y - ty;
-            const float dd = dx * dx;
-            const float ee = dy * dy;
-            const float d2 = dd + ee;
-            const bool contains = (p - l.x) < l.y;
-            const bool acc = !contains && d2 >= g.w;
-            if (kEvidence) {
-                if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++visited;  // a cell (a leaf: its first body)
-            }
-            if (acc) {
-                point_mass_d(dx, dy, d2, g.z, eps2, G, ax, ay);
-                if (kEvidence) {
-                    unsigned long long word;
-                    if (l.w & kTagBody) {
-                        ++direct;
-                        word = (1ull << 60) | (unsigned long long)(l.w & kTagValue);
-                    } else {
-                        ++accepted;
-                        word = (2ull << 60) | ((unsigned long long)(l.w & 255u) << 32) | l.x;
-                    }
-                    mix64(h, word);
-                }
-            }
-            c = acc ? max(l.z, c + 1u) : c + 1u;  // strictly increasing: the walk ends
-        }
+        WalkEvidence ev;
+        const uint32_t c0 = valid ? 0u : nn;
+        if (general)
+            walk_nodes<kEvidence, true, false>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
+                                               ax, ay, ev);
+        else if (fast)
+            walk_nodes<kEvidence, false, true>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
+                                               ax, ay, ev);
+        else
+            walk_nodes<kEvidence, false, false>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
+                                                ax, ay, ev);
+        __builtin_amdgcn_wave_barrier();
         if (valid) {
             const uint32_t body = S.val0[p];
             S.acc[body] = make_float2(ax, ay);
             if (kEvidence && tr.sim == ev_sim) {
                 if (ev_stats) {
-                    ev_stats[3 * (size_t)body + 0] = visited;
-                    ev_stats[3 * (size_t)body + 1] = accepted;
-                    ev_stats[3 * (size_t)body + 2] = direct;
+                    ev_stats[3 * (size_t)body + 0] = ev.visited;
+                    ev_stats[3 * (size_t)body + 1] = ev.accepted;
+                    ev_stats[3 * (size_t)body + 2] = ev.direct;
                 }
-                if (ev_hash) ev_hash[body] = h;
+                if (ev_hash) ev_hash[body] = ev.h;
             }
         }
     }

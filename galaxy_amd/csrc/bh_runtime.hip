@@ -61,14 +61,20 @@ static uint32_t div_up(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 // Upper bound of the cell count for n bodies: cells at depth d are distinct
 // 2d-bit key prefixes (<= 4^d) and disjoint non-empty ranges (<= n).
-uint64_t cell_capacity(uint32_t n) {
+constexpr uint64_t cell_capacity(uint32_t n) {
     uint64_t cap = 0, p4 = 1;
     for (uint32_t d = 0; d < kLevels; ++d) {
-        cap += std::min<uint64_t>(p4, n);
+        cap += p4 < n ? p4 : n;
         p4 *= 4;
     }
     return cap;
 }
+
+// The node stream (at most cell_capacity(n) + n nodes) must fit the subtree-size
+// field of the hot node word for every tree the library builds.
+static_assert(sizeof(NodeHot) == 16 && sizeof(NodeCold) == 16, "node records are 16 bytes");
+static_assert(cell_capacity(GBH_MAX_LARGE_TREE_BODIES) + GBH_MAX_LARGE_TREE_BODIES < kMaxNodes,
+              "the node stream of the largest tree fits the hot word");
 
 uint32_t span_capacity(uint32_t n) { return 16u * div_up(n, kTopoTile) + 16u; }
 
@@ -104,8 +110,10 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
     d.span = (uint32_t*)take(4ull * scap);
-    d.nodes = (NodeT*)take(sizeof(NodeT) * (cap + n));
+    d.nodes = (NodeHot*)take(sizeof(NodeHot) * (cap + n));
+    d.ncold = (NodeCold*)take(sizeof(NodeCold) * (cap + n));
     d.nbase = (uint32_t*)take(4ull * n);
+    d.bnode = (uint32_t*)take(4ull * n);
     d.bpart = (float4*)take(16ull * div_up(n, kTile));
     d.meta = (SimMeta*)take(sizeof(SimMeta));
     d.cell_cap = (uint32_t)cap;
