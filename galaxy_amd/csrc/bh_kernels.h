@@ -61,24 +61,31 @@ struct __attribute__((aligned(32))) CellT {
 // c + size, else at c + 1.  A body node's subtree is itself (size 1, T = 0), so
 // it interacts with every target but its own body.
 //
-// The traversal reads only the 16-byte hot record; T comes from a per-depth
-// table of the simulation (code 0..16 = depth, kCodeBody = the body marker with
-// T = 0).  The cold record carries the sorted range, the node's own threshold
-// and the evidence tag, for the evidence traversal and for host-built trees
-// whose ranges or thresholds do not follow the preorder (general mode).
+// The traversal reads only the 16-byte hot record; T comes from the
+// simulation's 16-entry code table (SimMeta::tab), with the code stored
+// pre-scaled by 4 so that it is the table's byte offset.  Code 0 has T = 0 and
+// marks bodies and the root (the root contains every target, so its threshold
+// is never used); codes 1..15 are depths 1..15 (depth-16 cells are always
+// leaves, flat.rs:127-177, so they never appear as internal nodes).  The cold
+// record carries the sorted range, the node's own threshold and the evidence
+// tag, for the evidence traversal and for host-built trees whose ranges or
+// thresholds do not follow the preorder (general mode).
 struct __attribute__((aligned(16))) NodeHot {
     float x, y, m;
-    uint32_t w;  // (subtree size << kSizeShift) | code
+    uint32_t w;  // (subtree size << kSizeShift) | (code << 2)
 };
 struct __attribute__((aligned(16))) NodeCold {
     uint32_t start, count;
     float T;
     uint32_t tag;  // kTagBody | body index, or the depth
 };
-constexpr uint32_t kSizeShift = 5;
-constexpr uint32_t kCodeMask = (1u << kSizeShift) - 1u;
-constexpr uint32_t kCodeBody = kCodeMask;           // T = 0
-constexpr uint32_t kMaxNodes = 1u << (32 - kSizeShift);  // subtree sizes fit the hot word
+constexpr uint32_t kSizeShift = 6;
+constexpr uint32_t kCodes = 16;
+constexpr uint32_t kCodeField = (kCodes - 1u) << 2;          // byte offset into the code table
+constexpr uint32_t kMaxNodes = 1u << (32 - kSizeShift);      // subtree sizes fit the hot word
+__host__ __device__ constexpr uint32_t node_word(uint32_t size, uint32_t code) {
+    return (size << kSizeShift) | (code << 2);
+}
 constexpr uint32_t kTagBody = 1u << 31;
 constexpr uint32_t kTagFirst = 1u << 30;  // first body of its leaf
 constexpr uint32_t kTagValue = kTagFirst - 1u;
@@ -90,7 +97,8 @@ struct Agg {  // f64 aggregates of a cell (mass, centre of mass)
 struct SimMeta {
     double root[4];         // f64 root (cx, cy, half) of root_bounds
     float bounds_f32[4];    // TreeMeta.bounds
-    float T[32];            // node-code thresholds: depth 0..16, kCodeBody -> 0
+    float T[20];            // per-depth acceptance thresholds (17 used)
+    float tab[kCodes];      // traversal code table: 0 -> 0, code k -> threshold of code k
     uint32_t mode;          // 0: preorder-consistent nodes and T table; 1: general (cold records)
     uint32_t geom_ok;       // every position lies in a box of half-width < 1e18
     uint32_t ncells;
@@ -105,6 +113,8 @@ struct __attribute__((aligned(16))) SimDesc {
     uint32_t n, cell_cap, bucket, maxd;
     float theta, eps, G, dt;
     uint32_t first_tile, first_sort_tile, first_topo_tile, span_cap;
+    uint32_t node_cap;  // node records allocated (the traversal reads at most index node_cap - 1)
+    uint32_t pad_[3];
     // resident state (SoA)
     float2* pos;
     float* mass;
@@ -611,9 +621,9 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
             // half at depth d is half * 2^-d exactly in f64; one rounding to f32
             double hd = half;
             for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-            M->T[threadIdx.x] = accept_threshold((float)hd, S.theta);
-        } else if (threadIdx.x < 32) {
-            M->T[threadIdx.x] = threadIdx.x == kCodeBody ? 0.0f : __uint_as_float(0x7fc00000u);
+            const float T = accept_threshold((float)hd, S.theta);
+            M->T[threadIdx.x] = T;
+            if (threadIdx.x < kCodes) M->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
         }
     }
     // Morton keys of this tile
@@ -924,7 +934,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
                 wx = wx + bm * (double)b.x;
                 wy = wy + bm * (double)b.y;
                 const uint32_t ni = nfirst + (q - p);
-                S.nodes[ni] = NodeHot{b.x, b.y, b.z, (1u << kSizeShift) | kCodeBody};
+                S.nodes[ni] = NodeHot{b.x, b.y, b.z, node_word(1u, 0u)};
                 S.ncold[ni] = NodeCold{q, 1u, 0.0f,
                                        kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue)};
                 S.bnode[q] = ni;
@@ -997,7 +1007,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             if (k + 1u < count) {  // internal: its node (aggregates follow below)
                 const uint32_t ni = nb + k;
                 const uint32_t skip = e < n ? S.nbase[e] : nnodes;
-                S.nodes[ni].w = ((skip - ni) << kSizeShift) | (dmin + k);
+                S.nodes[ni].w = node_word(skip - ni, dmin + k);  // internal depths are 0..15
                 S.ncold[ni] = NodeCold{p, e - p, S.meta->T[dmin + k], dmin + k};
             }
         }
@@ -1114,7 +1124,9 @@ __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
     if (threadIdx.x < kLevels) {
         double hd = S.meta->root[2];
         for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-        S.meta->T[threadIdx.x] = accept_threshold((float)hd, S.theta);
+        const float T = accept_threshold((float)hd, S.theta);
+        S.meta->T[threadIdx.x] = T;
+        if (threadIdx.x < kCodes) S.meta->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
     }
     if (threadIdx.x == 0) S.meta->theta_bits = __float_as_uint(S.theta);
 }
@@ -1163,16 +1175,23 @@ struct WalkEvidence {
 // kFast: every r2 = d2 + eps2 of the walk is finite and positive (the
 // simulation's positions lie in a box of half-width < 1e18 and
 // 0 < eps2 <= FLT_MAX), where point_mass_pos equals point_mass_d bit for bit.
+// The walk is software-pipelined: the next node's gather is issued before the
+// interaction of the current one is evaluated.  The cursor never passes nn
+// (a subtree ends inside the stream), and the node arrays hold at least nn + 1
+// records, so the gather at nn stays in bounds and its value is not used.
+// `tab` is the byte address of the wavefront's code table in LDS.
 template <bool kEvidence, bool kGeneral, bool kFast>
-__device__ __forceinline__ void walk_nodes(const NodeHot* __restrict__ nodes,
-                                           const NodeCold* __restrict__ ncold,
-                                           const float* __restrict__ tab, uint32_t c, uint32_t nn,
-                                           uint32_t p, uint32_t self, float tx, float ty, float eps2,
-                                           float G, float& ax, float& ay, WalkEvidence& ev) {
-    const GlobalF4 nh = (GlobalF4)(const void*)nodes;
-    const GlobalU4 nc = (GlobalU4)(const void*)ncold;
-    while (c < nn) {
-        const F4v g = nh[c];  // x, y, m, (size << kSizeShift) | code
+struct NodeWalk {
+    const __attribute__((address_space(1))) F4v* nh;
+    const __attribute__((address_space(1))) U4v* nc;
+    uint32_t tab, nn, p, self;
+    float tx, ty, eps2, G;
+    float ax = 0.f, ay = 0.f;
+    WalkEvidence ev;
+
+    // Evaluates node c (record g), issues the gather of the next node into gn
+    // and returns the next cursor.
+    __device__ __forceinline__ uint32_t step(uint32_t c, const F4v& g, F4v& gn) {
         const uint32_t w = __float_as_uint(g.w);
         const float dx = g.x - tx;
         const float dy = g.y - ty;
@@ -1189,9 +1208,11 @@ __device__ __forceinline__ void walk_nodes(const NodeHot* __restrict__ nodes,
             T = __uint_as_float(l.z);
         } else {
             inside = (self - c) < size;
-            T = tab[w & kCodeMask];
+            T = *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
         }
         const bool acc = !inside && d2 >= T;
+        const uint32_t next = c + (acc ? size : 1u);  // size >= 1: the walk ends
+        gn = nh[next];
         if (kEvidence) {
             if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++ev.visited;  // a cell (a leaf: its first body)
         }
@@ -1210,9 +1231,19 @@ __device__ __forceinline__ void walk_nodes(const NodeHot* __restrict__ nodes,
                 mix64(ev.h, word);
             }
         }
-        c += acc ? size : 1u;  // size >= 1: the walk ends
+        return next;
     }
-}
+
+    // Two steps per trip with the records alternating between g0 and g1.
+    __device__ __forceinline__ void run(uint32_t c) {
+        F4v g0 = nh[c], g1;
+        while (c < nn) {
+            c = step(c, g0, g1);
+            if (c >= nn) break;
+            c = step(c, g1, g0);
+        }
+    }
+};
 
 template <bool kEvidence>
 __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restrict__ sims,
@@ -1221,8 +1252,10 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
                                                         uint32_t* __restrict__ ev_stats,
                                                         unsigned long long* __restrict__ ev_hash,
                                                         uint32_t ev_sim) {
-    __shared__ float s_T[256 / 64][32];
-    float* tab = s_T[threadIdx.x >> 6];
+    __shared__ __attribute__((aligned(64))) float s_tab[256 / 64][kCodes];
+    float* tabp = s_tab[threadIdx.x >> 6];
+    const uint32_t tab = __builtin_amdgcn_readfirstlane(
+        (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)tabp);
     const uint32_t lane = lane_id();
     for (;;) {
         uint32_t item = 0;
@@ -1234,15 +1267,17 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const uint32_t n = S.n;
         const uint32_t p = tr.base + lane;
         const bool valid = p < n;
-        const uint32_t nn = __builtin_amdgcn_readfirstlane(S.meta->nnodes);
+        // a failed build (its error bits are reported by the host) cannot send
+        // the walk past the node arrays
+        const uint32_t nn = __builtin_amdgcn_readfirstlane(min(S.meta->nnodes, S.node_cap - 1u));
         const uint32_t general = __builtin_amdgcn_readfirstlane(S.meta->mode);
         const uint32_t geom_ok = __builtin_amdgcn_readfirstlane(S.meta->geom_ok);
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
         const bool fast = geom_ok != 0u && eps2 > 0.0f && eps2 <= 3.4028234663852886e38f;
-        // this wavefront's copy of the threshold table (LDS is in order per wavefront)
-        if (lane < 32) tab[lane] = S.meta->T[lane];
+        // this wavefront's copy of the code table (LDS is in order per wavefront)
+        if (lane < kCodes) tabp[lane] = S.meta->tab[lane];
         __builtin_amdgcn_wave_barrier();
         float tx = 0.f, ty = 0.f;
         uint32_t self = 0;
@@ -1255,15 +1290,21 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         float ax = 0.f, ay = 0.f;
         WalkEvidence ev;
         const uint32_t c0 = valid ? 0u : nn;
-        if (general)
-            walk_nodes<kEvidence, true, false>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
-                                               ax, ay, ev);
-        else if (fast)
-            walk_nodes<kEvidence, false, true>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
-                                               ax, ay, ev);
-        else
-            walk_nodes<kEvidence, false, false>(S.nodes, S.ncold, tab, c0, nn, p, self, tx, ty, eps2, G,
-                                                ax, ay, ev);
+        const GlobalF4 nh = (GlobalF4)(const void*)S.nodes;
+        const GlobalU4 nc = (GlobalU4)(const void*)S.ncold;
+        if (general) {
+            NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            wk.run(c0);
+            ax = wk.ax, ay = wk.ay, ev = wk.ev;
+        } else if (fast) {
+            NodeWalk<kEvidence, false, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            wk.run(c0);
+            ax = wk.ax, ay = wk.ay, ev = wk.ev;
+        } else {
+            NodeWalk<kEvidence, false, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            wk.run(c0);
+            ax = wk.ax, ay = wk.ay, ev = wk.ev;
+        }
         __builtin_amdgcn_wave_barrier();
         if (valid) {
             const uint32_t body = S.val0[p];

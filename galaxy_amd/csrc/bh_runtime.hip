@@ -118,6 +118,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.meta = (SimMeta*)take(sizeof(SimMeta));
     d.cell_cap = (uint32_t)cap;
     d.span_cap = scap;
+    d.node_cap = (uint32_t)(cap + n);
     return align_up(off, 256);
 }
 

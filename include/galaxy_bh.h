@@ -25,11 +25,11 @@ extern "C" {
 #define GBH_NO_CHILD 0xffffffffu
 /* nbody_parallel.rs:20 (MAX_PARALLEL_TREE_BODIES) */
 #define GBH_MAX_PARALLEL_TREE_BODIES 65536u
-/* Upper bound of the beyond-cap mode (gbh_tree_create_large): 2^24 bodies, so
- * that every node index of the traversal's node stream (at most the cell
- * capacity bound plus one node per body, < 2^27) fits its 27-bit field; about
- * 20 GB of device memory per simulation at the bound. */
-#define GBH_MAX_LARGE_TREE_BODIES (1u << 24)
+/* Upper bound of the beyond-cap mode (gbh_tree_create_large): 2^23 bodies, so
+ * that every subtree size of the traversal's node stream (at most the cell
+ * capacity bound plus one node per body, < 2^26) fits its 26-bit field; about
+ * 10 GB of device memory per simulation at the bound. */
+#define GBH_MAX_LARGE_TREE_BODIES (1u << 23)
 
 /* ---------------------------------------------------------------------------
  * Frozen f32/u32 transfer records, byte-identical to runtime/src/nbody_gpu.rs:22-83
