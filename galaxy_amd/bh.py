@@ -106,6 +106,39 @@ class NbodyGpu:
         _check(L.lib().gbh_accel_read(evolving._h, _p(out), None))
         return out
 
+    # -- FP64 diagnostics (measurement; no reference counterpart) --
+    def direct_f64(self, evolving: "EvolvingState", config: "Config", index) -> np.ndarray:
+        """f64 direct accelerations of the listed bodies (direct_acceleration_for,
+        nbody/src/lib.rs:315-329, over the state widened to f64); shape (k, 2)."""
+        idx = np.ascontiguousarray(index, np.uint32)
+        ax = np.zeros(len(idx), np.float64)
+        ay = np.zeros(len(idx), np.float64)
+        cfg = config.c()
+        _check(L.lib().gbh_diag_direct(evolving._h, C.byref(cfg), len(idx), _p(idx), _p(ax), _p(ay)))
+        return np.stack([ax, ay], axis=1)
+
+    def probe_error(self, evolving: "EvolvingState", config: "Config", accel, probes: int) -> dict:
+        """probe_error (nbody/src/lib.rs:370-395): RMS and max relative error of
+        `accel` (rows ax, ay, ...) against f64 direct summation on probe bodies
+        k*(N-1)/(probes-1)."""
+        n = evolving.count
+        probes = max(1, min(int(probes), n))
+        den = max(probes - 1, 1)
+        idx = (np.arange(probes, dtype=np.uint64) * np.uint64(n - 1)) // np.uint64(den)
+        exact = self.direct_f64(evolving, config, idx.astype(np.uint32))
+        a = np.asarray(accel, np.float64)[idx.astype(np.int64), :2]
+        denom = np.maximum(np.hypot(exact[:, 0], exact[:, 1]), 1e-30)
+        rel = np.hypot(a[:, 0] - exact[:, 0], a[:, 1] - exact[:, 1]) / denom
+        return {"rms_relative": float(np.sqrt(np.mean(rel * rel))), "max_relative": float(rel.max()),
+                "probes": probes}
+
+    def energy(self, evolving: "EvolvingState", softening: float, g: float) -> dict:
+        """Total energy (kinetic + Plummer-softened pair potential), Lz and kinetic
+        energy of the resident state, f64."""
+        out = np.zeros(3, np.float64)
+        _check(L.lib().gbh_diag_energy(evolving._h, float(softening), float(g), _p(out)))
+        return {"energy": float(out[0]), "lz": float(out[1]), "kinetic": float(out[2])}
+
     # -- BH #2B1 / #2B2 host-tree ingest (nbody_gpu.rs:640-938) --
     def traverse(self, settings, bodies, entries, cells) -> np.ndarray:
         settings = np.ascontiguousarray(settings, np.uint32)

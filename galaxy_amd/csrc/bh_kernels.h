@@ -152,6 +152,16 @@ struct TileRef {
     uint32_t base;
 };
 
+// The simulation descriptors are read-only while a kernel runs.  Reading them
+// through the constant address space lets the compiler treat the buffer pointers
+// they hold as global memory: global_* loads and stores instead of flat_* ones,
+// which would also count against the LDS/scalar wait counter and serialize store
+// latency into every LDS wait.
+typedef __attribute__((address_space(4))) const SimDesc* SimTable;
+__device__ __forceinline__ const SimDesc& sim_at(const SimDesc* sims, uint32_t i) {
+    return *(const SimDesc*)&((SimTable)sims)[i];
+}
+
 // ----------------------------------------------------------------------------
 // wavefront primitives (wave64, DPP within 16-lane rows + lane reads across rows)
 // ----------------------------------------------------------------------------
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
                                                   uint32_t ntopo, int mode) {
     __shared__ float s_red[4][kTile / 64];
     const TileRef tr = tiles[blockIdx.x];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t i = tr.base + threadIdx.x;
     const bool valid = i < S.n;
     if (blockIdx.x == 0 && threadIdx.x < 6) ctl[threadIdx.x] = 0u;  // tickets + queue
@@ -548,7 +558,7 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
     __shared__ double s_root[3];
     __shared__ uint32_t s_hist[4][kRadix];
     const TileRef tr = stiles[blockIdx.x];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     // reset this tile's look-back flags for the four radix passes
     for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
@@ -677,7 +687,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     __syncthreads();
     const uint32_t t = s_ticket;
     const TileRef tr = stiles[t];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     const uint32_t shift = 8u * pass;
     const uint32_t* __restrict__ kin = (pass & 1u) ? S.key1 : S.key0;
@@ -774,77 +784,195 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
 // ranges, geometry (f64, flat.rs:96-103), first-child links, leaf aggregates
 // (fill_aggregates leaf branch, flat.rs:181-195) and thresholds.
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t group_start(const uint32_t* __restrict__ K, uint32_t p,
-                                                uint32_t kp, uint32_t d) {
-    if (d == 0) return 0u;
-    // gallop backward: largest q <= p whose predecessor differs at depth d
-    uint32_t hi = p;       // same prefix
-    uint32_t step = 1;
-    uint32_t lo;           // candidate with different prefix (or -1 => 0)
-    for (;;) {
-        if (step > hi) { lo = 0xffffffffu; break; }
-        uint32_t q = hi - step;
-        if (!same_prefix(K[q], kp, d)) { lo = q; break; }
-        hi = q;
-        step <<= 1;
-    }
-    // answer in (lo, hi]; binary search for the first same-prefix position
-    uint32_t a = (lo == 0xffffffffu) ? 0u : lo + 1u;
-    uint32_t b = hi;
-    while (a < b) {
-        uint32_t mid = a + ((b - a) >> 1);
-        if (same_prefix(K[mid], kp, d)) b = mid; else a = mid + 1;
-    }
-    return a;
+// A position q starts a group at depth d iff dmin(q) <= d (dmin(0) = 0: the
+// simulation's first position starts a group at every depth; equal keys give
+// dmin = 17, no group).  So the end of the depth-d group containing p is the
+// next position after p with dmin <= d (or n), and its start the last position
+// at or before p with dmin <= d.  These queries are answered without searching
+// the keys: per-wavefront ballot masks of "dmin <= d" in LDS, per-wavefront
+// first/last boundaries within the tile, and per-tile summaries over the other
+// tiles of the simulation (k_topo_pre, k_topo_scan).
+constexpr uint32_t kTopoWaves = kTopoTile / 64;
+constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
+constexpr uint32_t kNone = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint32_t p, uint32_t kp) {
+    if (p == 0) return 0u;
+    const uint32_t kq = K[p - 1];
+    return kq == kp ? 17u : lcp2(kq, kp) + 1u;
 }
 
-__device__ __forceinline__ uint32_t group_end(const uint32_t* __restrict__ K, uint32_t p,
-                                              uint32_t kp, uint32_t d, uint32_t limit) {
-    // first q in (p, limit] whose key differs from kp at depth d; limit <= n
-    if (d == 0) return limit;
-    uint32_t lo = p;  // same prefix
-    uint32_t step = 1;
-    uint32_t hi;
-    for (;;) {
-        uint32_t q = p + step;
-        if (q >= limit) { hi = limit; break; }
-        if (!same_prefix(K[q], kp, d)) { hi = q; break; }
-        lo = q;
-        step <<= 1;
+// Per topology tile: dmin of every position (cdmin) and, per depth d, the first
+// boundary position in the tile and one past the last (tsum[t][d], tsum[t][17+d];
+// kNone / 0 when the tile has none).
+__global__ __launch_bounds__(kTopoTile) void k_topo_pre(const SimDesc* __restrict__ sims,
+                                                        const TileRef* __restrict__ ttiles,
+                                                        uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s_first[kLevels], s_last[kLevels];
+    const uint32_t t = blockIdx.x;
+    const TileRef tr = ttiles[t];
+    const SimDesc& S = sim_at(sims, tr.sim);
+    const uint32_t n = S.n;
+    const uint32_t p = tr.base + threadIdx.x;
+    const bool valid = p < n;
+    if (threadIdx.x < kLevels) {
+        s_first[threadIdx.x] = kNone;
+        s_last[threadIdx.x] = 0u;
     }
-    uint32_t a = lo + 1, b = hi;  // first differing in [a, b]
-    while (a < b) {
-        uint32_t mid = a + ((b - a) >> 1);
-        if (same_prefix(K[mid], kp, d)) a = mid + 1; else b = mid;
+    uint32_t dmin = 17u;
+    if (valid) {
+        dmin = dmin_of(S.key0, p, S.key0[p]);
+        S.cdmin[p] = (uint8_t)dmin;
     }
-    return a;
+    __syncthreads();
+    const uint32_t wbase = tr.base + (threadIdx.x & ~63u);
+    for (uint32_t d = 0; d < kLevels; ++d) {
+        const unsigned long long m = __ballot(valid && dmin <= d);
+        if (m && lane_id() == 0) {
+            atomicMin(&s_first[d], wbase + (uint32_t)__builtin_ctzll(m));
+            atomicMax(&s_last[d], wbase + 64u - (uint32_t)__builtin_clzll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kLevels) {
+        tsum[(size_t)t * kTopoSum + threadIdx.x] = s_first[threadIdx.x];
+        tsum[(size_t)t * kTopoSum + kLevels + threadIdx.x] = s_last[threadIdx.x];
+    }
 }
 
-__global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restrict__ sims,
+// Inclusive wavefront scan with max / min (all lanes active).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, o);
+        if (lane_id() >= (uint32_t)o) v = max(v, u);
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_min_rev(uint32_t v) {  // min over lanes >= self
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_down((int)v, o);
+        if (lane_id() + (uint32_t)o < 64u) v = min(v, u);
+    }
+    return v;
+}
+
+// Per simulation (one workgroup): for every tile and depth, one past the last
+// boundary before the tile (psum[t][d], 0: none) and the first boundary after it
+// (psum[t][17+d], n: none).  Wavefront w handles depths w, w + 16, ...
+__global__ __launch_bounds__(1024) void k_topo_scan(const SimDesc* __restrict__ sims,
+                                                    const uint32_t* __restrict__ tsum,
+                                                    uint32_t* __restrict__ psum) {
+    const SimDesc& S = sim_at(sims, blockIdx.x);
+    const uint32_t T = (S.n + kTopoTile - 1) / kTopoTile;
+    const uint32_t t0 = S.first_topo_tile;
+    const uint32_t lane = lane_id();
+    for (uint32_t d = threadIdx.x >> 6; d < kLevels; d += 1024 / 64) {
+        uint32_t carry = 0;  // forward: max of "last + 1" over earlier tiles
+        for (uint32_t c = 0; c < T; c += 64) {
+            const uint32_t i = c + lane;
+            const uint32_t v = i < T ? tsum[(size_t)(t0 + i) * kTopoSum + kLevels + d] : 0u;
+            const uint32_t incl = max(wave_incl_max(v), carry);
+            const uint32_t prev = (uint32_t)__shfl_up((int)incl, 1);
+            if (i < T) psum[(size_t)(t0 + i) * kTopoSum + d] = lane == 0 ? carry : prev;
+            carry = __builtin_amdgcn_readlane(incl, 63);
+        }
+        uint32_t rcarry = S.n;  // backward: min of "first" over later tiles
+        for (uint32_t c = ((T + 63) / 64) * 64; c > 0; c -= 64) {
+            const uint32_t i = c - 64 + lane;
+            const uint32_t v = i < T ? tsum[(size_t)(t0 + i) * kTopoSum + d] : kNone;
+            const uint32_t incl = min(wave_incl_min_rev(v), rcarry);
+            const uint32_t next = (uint32_t)__shfl_down((int)incl, 1);
+            if (i < T) psum[(size_t)(t0 + i) * kTopoSum + kLevels + d] = lane == 63 ? rcarry : next;
+            rcarry = __builtin_amdgcn_readlane(incl, 0);
+        }
+    }
+}
+
+// Boundary queries of one topology tile (see k_topo_pre).
+struct TileBounds {
+    unsigned long long mask[kTopoWaves][kLevels];  // bit l: position wbase + l has dmin <= d
+    uint32_t before[kTopoWaves][kLevels];          // one past the last boundary in earlier waves (0: none)
+    uint32_t after[kTopoWaves][kLevels];           // first boundary in later waves (kNone: none)
+    uint32_t pre[kLevels], post[kLevels];          // tile summaries of the other tiles
+
+    // Every thread of the tile calls this (block-wide barriers).
+    __device__ void build(uint32_t base, bool valid, uint32_t dmin, const uint32_t* __restrict__ ps) {
+        const uint32_t w = threadIdx.x >> 6;
+        for (uint32_t d = 0; d < kLevels; ++d) {
+            const unsigned long long m = __ballot(valid && dmin <= d);
+            if (lane_id() == 0) mask[w][d] = m;
+        }
+        if (threadIdx.x < kLevels) {
+            pre[threadIdx.x] = ps[threadIdx.x];
+            post[threadIdx.x] = ps[kLevels + threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x < kLevels) {
+            const uint32_t d = threadIdx.x;
+            uint32_t b = 0;
+            for (uint32_t k = 0; k < kTopoWaves; ++k) {
+                before[k][d] = b;
+                const unsigned long long m = mask[k][d];
+                if (m) b = base + k * 64u + 64u - (uint32_t)__builtin_clzll(m);
+            }
+            uint32_t a = kNone;
+            for (uint32_t k = kTopoWaves; k > 0; --k) {
+                after[k - 1][d] = a;
+                const unsigned long long m = mask[k - 1][d];
+                if (m) a = base + (k - 1) * 64u + (uint32_t)__builtin_ctzll(m);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Last boundary at depth d at or before position x (x in the tile).
+    __device__ __forceinline__ uint32_t prev(uint32_t base, uint32_t x, uint32_t d) const {
+        const uint32_t o = x - base, w = o >> 6, l = o & 63u;
+        const unsigned long long m = mask[w][d] & (l == 63u ? ~0ull : ((2ull << l) - 1ull));
+        if (m) return base + w * 64u + 63u - (uint32_t)__builtin_clzll(m);
+        const uint32_t b = before[w][d] ? before[w][d] : pre[d];
+        return b - 1u;  // position 0 is a boundary at every depth, so b >= 1
+    }
+
+    // First boundary at depth d after position x (x in the tile), or n.
+    __device__ __forceinline__ uint32_t next(uint32_t base, uint32_t x, uint32_t d) const {
+        const uint32_t o = x - base, w = o >> 6, l = o & 63u;
+        const unsigned long long m = l == 63u ? 0ull : (mask[w][d] & (~0ull << (l + 1u)));
+        if (m) return base + w * 64u + (uint32_t)__builtin_ctzll(m);
+        return after[w][d] != kNone ? after[w][d] : post[d];
+    }
+};
+
+__global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __restrict__ sims,
                                                           const TileRef* __restrict__ ttiles,
                                                           uint32_t* __restrict__ ctl,
-                                                          unsigned long long* __restrict__ tflags) {
+                                                          unsigned long long* __restrict__ tflags,
+                                                          const uint32_t* __restrict__ psum) {
     __shared__ uint32_t s_scan[kTopoTile / 64];
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
+    __shared__ TileBounds tb;
     if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
     __syncthreads();
     const uint32_t t = s_ticket;
     const TileRef tr = ttiles[t];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
     const uint32_t p = tr.base + threadIdx.x;
     const bool valid = p < n;
     const uint32_t kp = valid ? K[p] : 0u;
-    uint32_t dmin = 17, count = 0, dleaf = 0, pl = 0;
+    uint32_t dmin = valid ? (uint32_t)S.cdmin[p] : 17u;
+    tb.build(tr.base, valid, dmin, psum + (size_t)t * kTopoSum);
+    uint32_t count = 0, dleaf = 0, pl = 0;
     if (valid) {
-        dmin = (p == 0) ? 0u : ((K[p - 1] == kp) ? 17u : lcp2(K[p - 1], kp) + 1u);
         if (dmin <= maxd) {
             bool exists = true;
             if (dmin >= 1) {
                 const uint32_t dd = dmin - 1;
-                pl = group_start(K, p, kp, dd);
+                pl = tb.prev(tr.base, p, dd);
                 const uint32_t q = pl + bucket;
                 exists = (q < n) && same_prefix(K[q], kp, dd);
             }
@@ -894,7 +1022,6 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
     const uint32_t nb = base - (s_excl_cols + (mine >> 16)) + p;
     S.cbase[p] = base;
     S.nbase[p] = nb;
-    S.cdmin[p] = (uint8_t)dmin;
     S.ccount[p] = (uint8_t)count;
     S.cparent[p] = pl;
     if (count == 0) return;
@@ -911,11 +1038,9 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_count(const SimDesc* __restr
         cx = cx + ((q & 1u) ? h : -h);
         cy = cy + ((q & 2u) ? h : -h);
     }
-    uint32_t limit = n;
     for (uint32_t d = dmin; d <= dleaf; ++d) {
         const uint32_t idx = base + (d - dmin);
-        const uint32_t e = group_end(K, p, kp, d, limit);
-        limit = e;
+        const uint32_t e = tb.next(tr.base, p, d);
         const bool leaf = (d == dleaf);
         CellT c;
         c.start = p;
@@ -984,16 +1109,27 @@ __device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, ui
 // K5: skip links (next cell after each subtree), the remaining child links, and
 // bottom-up aggregates of every internal cell whose range lies inside this tile.
 // Internal cells that cross a tile boundary go to the simulation's span list.
+// The tile's cells are a contiguous preorder range [cb0, cb1); when it fits
+// kLinkCells, children links and f64 aggregates are kept in LDS, so each depth
+// round of the bottom-up pass is an LDS round trip, and only depths present in
+// the tile are visited.  Larger tiles use the same rounds through global memory.
 // ----------------------------------------------------------------------------
+constexpr uint32_t kLinkCells = 1536;
+
 __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
                                                          const TileRef* __restrict__ ttiles) {
+    __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];
+    __shared__ uint32_t s_ch[kLinkCells][4];
+    __shared__ uint32_t s_cb0, s_cb1, s_dlo, s_dhi;
     const TileRef tr = ttiles[blockIdx.x];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     const uint32_t p = tr.base + threadIdx.x;
     const bool valid = p < n;
     const uint32_t ncells = S.meta->ncells;
     const uint32_t nnodes = S.meta->nnodes;
+    const uint32_t tile_end = tr.base + kTopoTile;
+    const uint32_t last = min(n, tile_end) - 1u;
     uint32_t count = 0, dmin = 17, base = 0, nb = 0;
     if (valid) {
         count = S.ccount[p];
@@ -1001,41 +1137,94 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
         base = S.cbase[p];
         nb = S.nbase[p];
     }
-    if (count > 0 && base + count <= S.cell_cap) {
+    if (threadIdx.x == 0) {
+        s_cb0 = base;  // cells before the tile
+        s_dlo = 16u;
+        s_dhi = 0u;
+    }
+    if (p == last) s_cb1 = base + count;
+    bool ok = count > 0 && base + count <= S.cell_cap;
+    __syncthreads();
+    const uint32_t cb0 = s_cb0, cb1 = s_cb1;
+    const bool in_lds = cb0 <= cb1 && cb1 - cb0 <= kLinkCells;
+    // a consistent scan puts every column of the tile inside [cb0, cb1)
+    if (ok && in_lds && (base < cb0 || base + count > cb1)) {
+        atomicOr(&S.meta->err, kErrCellCapacity);
+        ok = false;
+    }
+    if (!ok) count = 0;
+    if (in_lds)
+        for (uint32_t i = threadIdx.x; i < cb1 - cb0; i += kTopoTile)
+            s_ch[i][0] = s_ch[i][1] = s_ch[i][2] = s_ch[i][3] = kNoChild;
+    __syncthreads();
+    uint32_t inside = 0;  // bit k: cell base + k is internal and lies inside the tile
+    if (ok) {
+        const uint32_t kp = S.key0[p];
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t e = S.cells[base + k].end;
+            const uint32_t d = dmin + k;
             if (k + 1u < count) {  // internal: its node (aggregates follow below)
                 const uint32_t ni = nb + k;
                 const uint32_t skip = e < n ? S.nbase[e] : nnodes;
-                S.nodes[ni].w = node_word(skip - ni, dmin + k);  // internal depths are 0..15
-                S.ncold[ni] = NodeCold{p, e - p, S.meta->T[dmin + k], dmin + k};
+                S.nodes[ni].w = node_word(skip - ni, d);  // internal depths are 0..15
+                S.ncold[ni] = NodeCold{p, e - p, S.meta->T[d], d};
+                if (e <= tile_end) inside |= 1u << k;
+                if (in_lds) s_ch[base + k - cb0][(kp >> (30u - 2u * d)) & 3u] = base + k + 1u;
+            } else if (in_lds) {  // the leaf's aggregate (k_topo_count)
+                const Agg a = S.cagg[base + k];
+                s_m[base + k - cb0] = a.m;
+                s_x[base + k - cb0] = a.x;
+                s_y[base + k - cb0] = a.y;
             }
+        }
+        if (count > 1u) {
+            atomicMin(&s_dlo, dmin);
+            atomicMax(&s_dhi, dmin + count - 2u);
         }
         if (dmin >= 1) {
             const uint32_t pl = S.cparent[p];
             const uint32_t parent = S.cbase[pl] + (dmin - 1u - S.cdmin[pl]);
-            const uint32_t kp = S.key0[p];
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
-            if (S.cdmin[pl] <= dmin - 1u && parent < ncells) (&S.cchild[parent].x)[q] = base;
-            else atomicOr(&S.meta->err, kErrCellCapacity);
+            if (S.cdmin[pl] <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
+                (&S.cchild[parent].x)[q] = base;
+                if (in_lds && parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = base;
+            } else {
+                atomicOr(&S.meta->err, kErrCellCapacity);
+            }
         }
-    } else {
-        count = 0;
     }
     __syncthreads();
-    const uint32_t tile_end = tr.base + kTopoTile;
-    const uint32_t dleaf = dmin + count - 1u;
-    for (int d = 15; d >= 0; --d) {
-        if (count > 0 && (uint32_t)d >= dmin && (uint32_t)d < dleaf) {
-            const uint32_t idx = base + ((uint32_t)d - dmin);
-            const uint32_t e = S.cells[idx].end;
-            if (e <= tile_end) {
-                const Agg a = combine_children(S.cagg, S.cchild[idx]);
+    const uint32_t dlo = s_dlo, dhi = s_dhi;
+    for (int d = (int)dhi; d >= (int)dlo; --d) {
+        const uint32_t k = (uint32_t)d - dmin;
+        if (count > 1u && (uint32_t)d >= dmin && k + 1u < count) {
+            const uint32_t idx = base + k;
+            if (inside & (1u << k)) {
+                Agg a;
+                if (in_lds) {
+                    double mass = 0.0, wx = 0.0, wy = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t c = s_ch[idx - cb0][q];
+                        if (c != kNoChild) {
+                            const double cm = s_m[c - cb0];
+                            mass = mass + cm;
+                            wx = wx + cm * s_x[c - cb0];
+                            wy = wy + cm * s_y[c - cb0];
+                        }
+                    }
+                    a = Agg{mass, wx / mass, wy / mass, 0.0};
+                    s_m[idx - cb0] = a.m;
+                    s_x[idx - cb0] = a.x;
+                    s_y[idx - cb0] = a.y;
+                } else {
+                    a = combine_children(S.cagg, S.cchild[idx]);
+                }
                 S.cagg[idx] = a;
                 S.cells[idx].comx = (float)a.x;
                 S.cells[idx].comy = (float)a.y;
                 S.cells[idx].mass = (float)a.m;
-                NodeHot* nd = &S.nodes[nb + ((uint32_t)d - dmin)];
+                NodeHot* nd = &S.nodes[nb + k];
                 nd->x = (float)a.x;
                 nd->y = (float)a.y;
                 nd->m = (float)a.m;
@@ -1055,46 +1244,80 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
 
 // ----------------------------------------------------------------------------
 // K6: aggregates of the spanning internal cells (the top of the tree), one
-// workgroup per simulation, bottom-up by depth with the running values in LDS.
+// workgroup per simulation, bottom-up by depth.  Every global read happens
+// before the depth rounds: each spanning cell's children become references to
+// either another spanning cell's running value or a preloaded aggregate of a
+// non-spanning child, all in LDS, so a round is an LDS round trip.  Lists too
+// long for that (more than kTopLds / 5 spanning cells) are aggregated through
+// global memory in the same order.
 // ----------------------------------------------------------------------------
 constexpr uint32_t kTopThreads = 1024;
+constexpr uint32_t kTopLds = 4096;  // aggregate slots: spanning cells, then preloaded children
 
 __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims) {
-    __shared__ double s_m[kSpanSlotsLds], s_x[kSpanSlotsLds], s_y[kSpanSlotsLds];
-    __shared__ uint8_t s_depth[kSpanSlotsLds];
-    const SimDesc& S = sims[blockIdx.x];
+    __shared__ double s_m[kTopLds], s_x[kTopLds], s_y[kTopLds];
+    __shared__ uint32_t s_ref[kTopLds / 5][4];
+    __shared__ uint8_t s_depth[kTopLds / 5];
+    __shared__ uint32_t s_next;
+    const SimDesc& S = sim_at(sims, blockIdx.x);
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
-    const bool in_lds = nspan <= kSpanSlotsLds;
-    for (uint32_t i = threadIdx.x; i < nspan && in_lds; i += kTopThreads)
-        s_depth[i] = (uint8_t)(S.cells[S.span[i]].info & 255u);
+    const bool in_lds = nspan <= kTopLds / 5;
+    if (threadIdx.x == 0) s_next = nspan;
     __syncthreads();
+    if (in_lds) {
+        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+            const uint32_t idx = S.span[i];
+            s_depth[i] = (uint8_t)(S.cells[idx].info & 255u);
+            const uint4 ch = S.cchild[idx];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = (&ch.x)[q];
+                uint32_t r = kNoChild;
+                if (c != kNoChild) {
+                    const uint32_t slot = S.cells[c].info >> kInfoSpanShift;
+                    if (slot != 0u) {
+                        r = slot - 1u;
+                    } else {
+                        r = atomicAdd(&s_next, 1u);
+                        const Agg a = S.cagg[c];
+                        s_m[r] = a.m;
+                        s_x[r] = a.x;
+                        s_y[r] = a.y;
+                    }
+                }
+                s_ref[i][q] = r;
+            }
+        }
+        __syncthreads();
+    }
     for (int d = 15; d >= 0; --d) {
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
             const uint32_t idx = S.span[i];
             const uint32_t depth = in_lds ? s_depth[i] : (S.cells[idx].info & 255u);
             if (depth != (uint32_t)d) continue;
-            const uint4 ch = S.cchild[idx];
             double mass = 0.0, wx = 0.0, wy = 0.0;
+            if (in_lds) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t c = (&ch.x)[q];
-                if (c == kNoChild) continue;
-                double cm, cxx, cyy;
-                const uint32_t slot = S.cells[c].info >> kInfoSpanShift;
-                if (in_lds && slot != 0u) {
-                    cm = s_m[slot - 1];
-                    cxx = s_x[slot - 1];
-                    cyy = s_y[slot - 1];
-                } else {
-                    const Agg a = S.cagg[c];
-                    cm = a.m;
-                    cxx = a.x;
-                    cyy = a.y;
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t r = s_ref[i][q];
+                    if (r == kNoChild) continue;
+                    const double cm = s_m[r];
+                    mass = mass + cm;
+                    wx = wx + cm * s_x[r];
+                    wy = wy + cm * s_y[r];
                 }
-                mass = mass + cm;
-                wx = wx + cm * cxx;
-                wy = wy + cm * cyy;
+            } else {
+                const uint4 ch = S.cchild[idx];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t c = (&ch.x)[q];
+                    if (c == kNoChild) continue;
+                    const Agg a = S.cagg[c];
+                    mass = mass + a.m;
+                    wx = wx + a.m * a.x;
+                    wy = wy + a.m * a.y;
+                }
             }
             const double comx = wx / mass, comy = wy / mass;
             if (in_lds) {
@@ -1120,7 +1343,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
 __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
     const uint32_t s = blockIdx.x;
     if (s >= nsims) return;
-    const SimDesc& S = sims[s];
+    const SimDesc& S = sim_at(sims, s);
     if (threadIdx.x < kLevels) {
         double hd = S.meta->root[2];
         for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
@@ -1132,7 +1355,7 @@ __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
 }
 
 __global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
-    const SimDesc& S = sims[sim];
+    const SimDesc& S = sim_at(sims, sim);
     const uint32_t nc = S.meta->ncells;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
         S.cells[i].T = S.meta->T[S.cells[i].info & 255u];
@@ -1263,7 +1486,7 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         item = __builtin_amdgcn_readlane(item, 0);
         if (item >= nwtiles) break;
         const TileRef tr = wtiles[item];
-        const SimDesc& S = sims[tr.sim];
+        const SimDesc& S = sim_at(sims, tr.sim);
         const uint32_t n = S.n;
         const uint32_t p = tr.base + lane;
         const bool valid = p < n;
@@ -1327,7 +1550,7 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
 __global__ __launch_bounds__(kTile) void k_final_kick(const SimDesc* __restrict__ sims,
                                                       const TileRef* __restrict__ tiles) {
     const TileRef tr = tiles[blockIdx.x];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t i = tr.base + threadIdx.x;
     if (i >= S.n) return;
     const float h = 0.5f * S.dt;
@@ -1342,7 +1565,7 @@ __global__ __launch_bounds__(kTile) void k_final_kick(const SimDesc* __restrict_
 __global__ __launch_bounds__(kTile) void k_kick_drift(const SimDesc* __restrict__ sims,
                                                       const TileRef* __restrict__ tiles) {
     const TileRef tr = tiles[blockIdx.x];
-    const SimDesc& S = sims[tr.sim];
+    const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t i = tr.base + threadIdx.x;
     if (i >= S.n) return;
     const float dt = S.dt;
@@ -1356,6 +1579,55 @@ __global__ __launch_bounds__(kTile) void k_kick_drift(const SimDesc* __restrict_
     p.y = p.y + dt * hvy;
     S.pos[i] = p;
     S.vel[i] = make_float2(hvx, hvy);
+}
+
+// FP64 direct-summation diagnostics over the resident state (measurement only,
+// not on the step path).  k_diag_direct: the acceleration of each listed body,
+// summed over all other bodies in index order with the operation order of
+// add_point_mass / direct_acceleration_for (nbody/src/lib.rs:247-255, 315-329).
+// k_diag_potential: per body i, -G m_i sum_{j > i} m_j / sqrt(r2 + eps2), the
+// j-sum in index order.  f64 sqrt and division are correctly rounded.
+__global__ __launch_bounds__(256) void k_diag_direct(const float2* __restrict__ pos,
+                                                     const float* __restrict__ mass, uint32_t n,
+                                                     const uint32_t* __restrict__ index,
+                                                     uint32_t count, double eps2, double G,
+                                                     double* __restrict__ ax,
+                                                     double* __restrict__ ay) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t i = index[k];
+    const double tx = (double)pos[i].x, ty = (double)pos[i].y;
+    double sx = 0.0, sy = 0.0;
+    for (uint32_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const float2 q = pos[j];
+        const double dx = (double)q.x - tx;
+        const double dy = (double)q.y - ty;
+        const double r2 = dx * dx + dy * dy + eps2;
+        const double inv_r = 1.0 / __builtin_sqrt(r2);
+        const double scale = G * (double)mass[j] * inv_r * inv_r * inv_r;
+        sx += dx * scale;
+        sy += dy * scale;
+    }
+    ax[k] = sx;
+    ay[k] = sy;
+}
+
+__global__ __launch_bounds__(256) void k_diag_potential(const float2* __restrict__ pos,
+                                                        const float* __restrict__ mass, uint32_t n,
+                                                        double eps2, double G,
+                                                        double* __restrict__ pot) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = (double)pos[i].x, yi = (double)pos[i].y;
+    double s = 0.0;
+    for (uint32_t j = i + 1; j < n; ++j) {
+        const float2 q = pos[j];
+        const double dx = (double)q.x - xi;
+        const double dy = (double)q.y - yi;
+        s += (double)mass[j] / __builtin_sqrt(dx * dx + dy * dy + eps2);
+    }
+    pot[i] = -G * (double)mass[i] * s;
 }
 
 // Streaming copy used to calibrate the HBM traffic counters.

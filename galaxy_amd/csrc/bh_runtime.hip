@@ -151,8 +151,10 @@ struct SimSet {
     uint32_t* d_ctl = nullptr;
     uint32_t* d_sflags = nullptr;
     unsigned long long* d_tflags = nullptr;
+    uint32_t* d_tsum = nullptr;  // per topology tile boundary summaries (k_topo_pre)
+    uint32_t* d_psum = nullptr;  // and their scans over the simulation's tiles (k_topo_scan)
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
-    size_t cap_sflags = 0, cap_tflags = 0;
+    size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0;
     uint32_t sort_kpt = 8;
     uint32_t traverse_blocks = 0;
 
@@ -160,13 +162,15 @@ struct SimSet {
 
     void release() {
         for (void* p : {(void*)d_desc, (void*)d_t256, (void*)d_tsort, (void*)d_ttopo,
-                        (void*)d_twave, (void*)d_ctl, (void*)d_sflags, (void*)d_tflags})
+                        (void*)d_twave, (void*)d_ctl, (void*)d_sflags, (void*)d_tflags,
+                        (void*)d_tsum, (void*)d_psum})
             if (p) (void)hipFree(p);
         d_desc = nullptr;
         d_t256 = d_tsort = d_ttopo = d_twave = nullptr;
-        d_ctl = d_sflags = nullptr;
+        d_ctl = d_sflags = d_tsum = d_psum = nullptr;
         d_tflags = nullptr;
         cap_desc = cap_t256 = cap_tsort = cap_ttopo = cap_twave = cap_sflags = cap_tflags = 0;
+        cap_tsum = cap_psum = 0;
     }
 
     template <class T>
@@ -216,6 +220,8 @@ struct SimSet {
         ensure(d_ctl, ctl_cap, kCtlWords);
         ensure(d_sflags, cap_sflags, 4ull * kRadix * tsort.size());
         ensure(d_tflags, cap_tflags, ttopo.size());
+        ensure(d_tsum, cap_tsum, kTopoSum * ttopo.size());
+        ensure(d_psum, cap_psum, kTopoSum * ttopo.size());
         HIP_OK(hipMemcpyAsync(d_desc, desc.data(), sizeof(SimDesc) * desc.size(),
                               hipMemcpyHostToDevice, stream));
         auto put = [&](TileRef* dst, const std::vector<TileRef>& v) {
@@ -270,8 +276,14 @@ struct SimSet {
         }
     }
     void topo(hipStream_t s) {
+        hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
+                           d_ttopo, d_tsum);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size()), dim3(1024), 0, s, d_desc, d_tsum,
+                           d_psum);
+        HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_count, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
-                           d_ttopo, d_ctl, d_tflags);
+                           d_ttopo, d_ctl, d_tflags, d_psum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_link, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
                            d_ttopo);
@@ -1049,6 +1061,81 @@ int gbh_copy_calibration(gbh_device* dev, uint64_t bytes, double* ms) {
         (void)hipEventDestroy(b);
         HIP_OK(hipFree(src));
         HIP_OK(hipFree(dst));
+        return 0;
+    });
+}
+
+int gbh_diag_direct(gbh_state* st, const gbh_config* cfg, uint32_t count, const uint32_t* index,
+                    double* ax, double* ay) {
+    return guarded([&] {
+        if (!st || !cfg || (count && (!index || !ax || !ay))) {
+            set_error("null argument");
+            return -1;
+        }
+        if (!validate_config(cfg)) return -1;
+        for (uint32_t k = 0; k < count; ++k)
+            if (index[k] >= st->n) {
+                set_error("body index is out of range");
+                return -1;
+            }
+        if (count == 0) return 0;
+        hipStream_t s = st->dev->stream;
+        const SimDesc& d = st->kdk.desc[0];
+        uint32_t* d_idx = nullptr;
+        double* d_out = nullptr;
+        HIP_OK(hipMalloc((void**)&d_idx, 4ull * count));
+        HIP_OK(hipMalloc((void**)&d_out, 16ull * count));
+        HIP_OK(hipMemcpyAsync(d_idx, index, 4ull * count, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_diag_direct, dim3(div_up(count, 256)), dim3(256), 0, s, d.pos, d.mass,
+                           st->n, d_idx, count, cfg->softening * cfg->softening, cfg->g, d_out,
+                           d_out + count);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(ax, d_out, 8ull * count, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(ay, d_out + count, 8ull * count, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipFree(d_idx));
+        HIP_OK(hipFree(d_out));
+        return 0;
+    });
+}
+
+int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]) {
+    return guarded([&] {
+        if (!st || !out) {
+            set_error("null argument");
+            return -1;
+        }
+        if (!std::isfinite(softening) || !std::isfinite(g)) {
+            set_error("softening and G must be finite");
+            return -1;
+        }
+        hipStream_t s = st->dev->stream;
+        const SimDesc& d = st->kdk.desc[0];
+        const uint32_t n = st->n;
+        double* d_pot = nullptr;
+        HIP_OK(hipMalloc((void**)&d_pot, 8ull * n));
+        hipLaunchKernelGGL(k_diag_potential, dim3(div_up(n, 256)), dim3(256), 0, s, d.pos, d.mass, n,
+                           softening * softening, g, d_pot);
+        HIP_OK(hipGetLastError());
+        std::vector<double> pot(n);
+        std::vector<float2> pos(n), vel(n);
+        std::vector<float> mass(n);
+        HIP_OK(hipMemcpyAsync(pot.data(), d_pot, 8ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(pos.data(), d.pos, 8ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(vel.data(), d.vel, 8ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(mass.data(), d.mass, 4ull * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipFree(d_pot));
+        double kin = 0.0, potential = 0.0, lz = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const double m = mass[i], x = pos[i].x, y = pos[i].y, vx = vel[i].x, vy = vel[i].y;
+            kin += 0.5 * m * (vx * vx + vy * vy);
+            potential += pot[i];
+            lz += m * (x * vy - y * vx);
+        }
+        out[0] = kin + potential;
+        out[1] = lz;
+        out[2] = kin;
         return 0;
     });
 }

@@ -252,6 +252,15 @@ int gbh_server_take_result(gbh_server* s, uint64_t id, GbhEvolveState* out, GbhG
 /* queued, in-flight, resident bytes */
 int gbh_server_stats(gbh_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* resident);
 
+/* FP64 diagnostics over a resident state (no reference counterpart; the
+ * measurement side of the driver's oracle checks).  gbh_diag_direct: f64 direct
+ * accelerations of the listed bodies, as direct_acceleration_for
+ * (nbody/src/lib.rs:315-329) on the state widened to f64; probe_error
+ * (lib.rs:370-395) is computed from them by the caller.  gbh_diag_energy:
+ * out = {kinetic + Plummer-softened pair potential, Lz, kinetic}, f64. */
+int gbh_diag_direct(gbh_state* st, const gbh_config* cfg, uint32_t count, const uint32_t* index,
+                    double* ax, double* ay);
+int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]);
 /* Exhaustive device check (no reference counterpart) that the traversal's f32
  * square root and reciprocal are correctly rounded for every positive f32:
  * mismatches[0] = sqrt, [1] = reciprocal; first_bad = smallest failing pattern. */
