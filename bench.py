@@ -84,12 +84,23 @@ def plain_run(gpu, steps, warmup):
                 max_depth=maxd)
 
 
-def batched_run(gpu, steps, warmup, count=256):
-    from galaxy_amd import ics
+def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None):
+    from dataclasses import asdict, replace
+
+    from galaxy_amd import autotune, ics
     from galaxy_amd.batch import Batch
 
     w = ics.config3(count=count)
-    b = Batch(gpu, w.states, w.config, w.dt, use_graph=True, sort_tile=2048)
+    tuning = None
+    choice = autotune.Choice()
+    if tune:
+        log("  autotuning leaf size / sort tile / traversal launch shape ...")
+        tuning = autotune.tune(gpu, w.states, w.config, w.dt, cache=tune_cache, steps=5, log=log)
+        choice = tuning["choice"]
+        log(f"  tuned: {asdict(choice)}")
+    cfg = replace(w.config, bucket=choice.bucket)
+    b = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=choice.sort_tile,
+              traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu)
     total = w.bodies
     b.step(max(warmup, 1))
     t0 = time.perf_counter()
@@ -103,13 +114,18 @@ def batched_run(gpu, steps, warmup, count=256):
         st, _ = b.force_evidence(i)
         inter += int(st[:, 1].sum() + st[:, 2].sum())
     b.close()
+    tune_out = None
+    if tuning is not None:
+        tune_out = {"choice": asdict(choice), "cached": tuning["cached"],
+                    "measured_ms_per_step": [dict(c, ms=1e3 * t) for c, t in tuning["measured"]]}
     return dict(sims=len(w.states), bodies=total, steps=steps, seconds=el,
                 body_steps_per_s=total * steps / el, ms_per_step=1e3 * el / steps, phase_ms=avg,
-                interactions=inter, device_bytes=0)
+                interactions=inter, choice=asdict(choice), tuning=tune_out)
 
 
-def serving_run(gpu, requests=300, rate_lo=50.0, rate_hi=2000.0, seed=777):
-    """Bounded sample of the serving stream: Poisson arrivals ramping rate_lo -> rate_hi."""
+def serving_run(gpu, requests=300, rate_lo=50.0, rate_hi=2000.0, seed=777, choice=None):
+    """Bounded sample of the serving stream: Poisson arrivals ramping rate_lo -> rate_hi.
+    Requests carry their own physics (leaf size 4); the server uses the tuned launch shape."""
     from galaxy_amd import Config, ics
     from galaxy_amd.serve import Server
 
@@ -131,7 +147,8 @@ def serving_run(gpu, requests=300, rate_lo=50.0, rate_hi=2000.0, seed=777):
             s = ics.uniform_collapse(n, 10**6 + k)
             cfg, dt = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16), 5e-4
         reqs.append((at, s, cfg, dt, steps))
-    srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10)
+    shape = {k: choice[k] for k in ("sort_tile", "traverse_block", "traverse_waves_per_cu")} if choice else {}
+    srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10, **shape)
     t0 = srv.now()
     i, done = 0, []
     body_steps = sum(len(s) * st for _, s, _, _, st in reqs)
@@ -156,7 +173,7 @@ def serving_run(gpu, requests=300, rate_lo=50.0, rate_hi=2000.0, seed=777):
                 failed=int(sum(d.status != 0 for d in done)))
 
 
-def cpu_baseline(steps=8):
+def cpu_baseline(steps=40):
     """Oracle (restated reference CPU path, f64 BH #2A cpu_flat_evolve), single thread."""
     from galaxy_amd import ics
     from oracle import cpu_ref
@@ -182,6 +199,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--no-tune", action="store_true", help="batched leg at the default launch shape and leaf size 4")
+    ap.add_argument("--tune-cache", default=None, help="JSON cache of autotuner results (default: none, tune every run)")
     ap.add_argument("--only", choices=["plain", "batched", "copy"], default=None,
                     help="run one leg only (for profiling); prints that leg's result")
     args = ap.parse_args()
@@ -211,18 +230,19 @@ def main():
         print(json.dumps(plain_run(gpu, args.steps, args.warmup)), flush=True)
         return
     if args.only == "batched":
-        print(json.dumps(batched_run(gpu, args.steps, args.warmup)), flush=True)
+        print(json.dumps(batched_run(gpu, args.steps, args.warmup, tune=not args.no_tune,
+                                     tune_cache=args.tune_cache)), flush=True)
         return
     log("plain drop-in, cold collapse N=65536 ...")
     plain = plain_run(gpu, args.steps, args.warmup)
     log(f"  plain {plain['body_steps_per_s']:.3e} body-steps/s, {plain['ms_per_step']:.3f} ms/step")
     log("batched ensemble, 256 discs ...")
-    batched = batched_run(gpu, args.steps, args.warmup)
+    batched = batched_run(gpu, args.steps, args.warmup, tune=not args.no_tune, tune_cache=args.tune_cache)
     log(f"  batched {batched['body_steps_per_s']:.3e} body-steps/s, {batched['ms_per_step']:.3f} ms/step")
     serving = None
     if not args.no_serving:
         log("serving sample ...")
-        serving = serving_run(gpu, requests=args.serving_requests)
+        serving = serving_run(gpu, requests=args.serving_requests, choice=batched["choice"])
         log(f"  serving p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s")
     cpu = None
     if not args.no_cpu:
@@ -248,12 +268,15 @@ def main():
         "data": "synthetic (seeded discs / cold collapse)",
         "config": {
             "workload": "batched ensemble of 256 rotating exponential discs, N log-uniform in [512, 65536] "
-                        f"({batched['bodies']} bodies), theta 0.5, eps 0.01, dt 1e-3, bucket 4; "
+                        f"({batched['bodies']} bodies), theta 0.5, eps 0.01, dt 1e-3, "
+                        f"autotuned leaf size {batched['choice']['bucket']}; "
                         "plain drop-in: cold collapse N=65536, theta 0.5, eps 0.005, dt 5e-4, bucket 4",
             "sims": batched["sims"],
             "bodies": batched["bodies"],
             "theta": 0.5,
+            "batched_choice": batched["choice"],
         },
+        "autotune": batched["tuning"],
         "plain_body_steps_per_s": plain["body_steps_per_s"],
         "plain_ms_per_step": plain["ms_per_step"],
         "batched_vs_plain": batched["body_steps_per_s"] / plain["body_steps_per_s"],

@@ -127,7 +127,6 @@ struct __attribute__((aligned(16))) SimDesc {
     uint32_t* val1;
     // sorted bodies
     float4* sbody;     // (x, y, m, 0) in sorted order
-    uint32_t* pos_of;  // body -> sorted position
     // topology per sorted position
     uint32_t* cbase;   // preorder index of the column's first cell
     uint32_t* cparent; // start of the parent group of the column head
@@ -397,6 +396,16 @@ __device__ __forceinline__ double rcp_f64_step(float sf) {
 
 __device__ __forceinline__ float rcp_rn_pos(float sf) { return (float)rcp_f64_step(sf); }
 
+// Reciprocal of a normal f32 below 2^126 (1/s normal): f32 hardware reciprocal
+// plus one Newton step with an exact residual (Markstein).  k_check_rounding
+// verifies it equals the correctly rounded division for every such input.
+constexpr uint32_t kRcpNormalLo = 0x00800000u, kRcpNormalHi = 0x7e800000u;  // [2^-126, 2^126)
+__device__ __forceinline__ float rcp_rn_normal(float s) {
+    const float y = __builtin_amdgcn_rcpf(s);
+    const float e = __builtin_fmaf(-s, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
+}
+
 // Reciprocal of a non-negative f32; 1/0 = +inf and 1/inf = 0 as for the division.
 __device__ __forceinline__ float rcp_rn(float sf) {
     const bool finite_pos = sf > 0.0f && sf < __builtin_inff();
@@ -418,13 +427,14 @@ __device__ __forceinline__ void point_mass_d(float dx, float dy, float d2, float
     ay = ay + dy * scale;
 }
 
-// The same where r2 = d2 + eps2 is known finite and positive (then sqrt_rn and
-// rcp_rn take their finite_pos branch, so the result is identical).
+// The same where r2 = d2 + eps2 is finite and positive: then sqrt_rn takes its
+// finite_pos branch and s lies in [2^-75, 2^64], where rcp_rn_normal is the
+// correctly rounded reciprocal, so the result is identical.
 __device__ __forceinline__ void point_mass_pos(float dx, float dy, float d2, float m, float eps2,
                                                float G, float& ax, float& ay) {
     float r2 = d2 + eps2;
     float s = sqrt_rn_pos(r2);
-    float inv_r = rcp_rn_pos(s);
+    float inv_r = rcp_rn_normal(s);
     float scale = G * m;
     scale = scale * inv_r;
     scale = scale * inv_r;
@@ -442,9 +452,10 @@ __device__ __forceinline__ void point_mass(float tx, float ty, float sx, float s
     point_mass_d(dx, dy, dd + ee, m, eps2, G, ax, ay);
 }
 
-// Exhaustive check of sqrt_rn / rcp_rn against the compiler's correctly rounded
-// sqrtf and division over every non-negative f32 bit pattern (+0 to +inf).
-// counts[0..1]: mismatches; first[0..1]: smallest mismatching bit pattern.
+// Exhaustive check of the traversal's rounding primitives against the
+// compiler's correctly rounded sqrtf and division: sqrt_rn and rcp_rn over every
+// non-negative f32 bit pattern (+0 to +inf), rcp_rn_normal over [2^-126, 2^126).
+// counts[0..2]: mismatches; first[0..2]: smallest mismatching bit pattern.
 __global__ void k_check_rounding(unsigned long long* __restrict__ counts,
                                  uint32_t* __restrict__ first) {
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -454,9 +465,15 @@ __global__ void k_check_rounding(unsigned long long* __restrict__ counts,
             atomicAdd(&counts[0], 1ull);
             atomicMin(&first[0], b);
         }
-        if (__float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x)) {
+        const float want = 1.0f / x;
+        if (__float_as_uint(rcp_rn(x)) != __float_as_uint(want)) {
             atomicAdd(&counts[1], 1ull);
             atomicMin(&first[1], b);
+        }
+        if (b >= kRcpNormalLo && b < kRcpNormalHi &&
+            __float_as_uint(rcp_rn_normal(x)) != __float_as_uint(want)) {
+            atomicAdd(&counts[2], 1ull);
+            atomicMin(&first[2], b);
         }
         if (b > 0x7f800000u - stride) break;
     }
@@ -768,10 +785,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
         const uint32_t g = s_glob[dd] + (l - s_start[dd]);
         kout[g] = kk;
         vout[g] = vv;
-        if (last_pass) {  // sorted body copy + target positions (bh_parallel_tree_positions)
+        if (last_pass) {  // sorted body copy
             const float2 p = S.pos[vv];
             S.sbody[g] = make_float4(p.x, p.y, S.mass[vv], 0.0f);
-            S.pos_of[vv] = g;
         }
     }
 }
@@ -954,6 +970,10 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
     __shared__ TileBounds tb;
+    // the tile's keys, sorted bodies and body indices, staged once with coalesced
+    // loads: existence, leaf-depth and leaf-body reads inside the tile come from LDS
+    __shared__ uint32_t s_key[kTopoTile], s_val[kTopoTile];
+    __shared__ float4 s_body[kTopoTile];
     if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
     __syncthreads();
     const uint32_t t = s_ticket;
@@ -965,7 +985,14 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
     const bool valid = p < n;
     const uint32_t kp = valid ? K[p] : 0u;
     uint32_t dmin = valid ? (uint32_t)S.cdmin[p] : 17u;
-    tb.build(tr.base, valid, dmin, psum + (size_t)t * kTopoSum);
+    if (valid) {
+        s_key[threadIdx.x] = kp;
+        s_val[threadIdx.x] = S.val0[p];
+        s_body[threadIdx.x] = S.sbody[p];
+    }
+    const uint32_t tile_end = min(tr.base + kTopoTile, n);
+    auto key_at = [&](uint32_t q) { return q - tr.base < tile_end - tr.base ? s_key[q - tr.base] : K[q]; };
+    tb.build(tr.base, valid, dmin, psum + (size_t)t * kTopoSum);  // its barriers publish the staging
     uint32_t count = 0, dleaf = 0, pl = 0;
     if (valid) {
         if (dmin <= maxd) {
@@ -974,10 +1001,10 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
                 const uint32_t dd = dmin - 1;
                 pl = tb.prev(tr.base, p, dd);
                 const uint32_t q = pl + bucket;
-                exists = (q < n) && same_prefix(K[q], kp, dd);
+                exists = (q < n) && same_prefix(key_at(q), kp, dd);
             }
             if (exists) {
-                const int L = (p + bucket < n) ? (int)lcp2(kp, K[p + bucket]) : -1;
+                const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
                 int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
                 if (dl < (int)dmin) dl = (int)dmin;
                 dleaf = (uint32_t)dl;
@@ -1053,7 +1080,9 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
             double mass = 0.0, wx = 0.0, wy = 0.0;
             const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
             for (uint32_t q = p; q < e; ++q) {
-                const float4 b = S.sbody[q];
+                const bool here = q < tile_end;
+                const float4 b = here ? s_body[q - tr.base] : S.sbody[q];
+                const uint32_t body = here ? s_val[q - tr.base] : S.val0[q];
                 const double bm = b.z;
                 mass = mass + bm;
                 wx = wx + bm * (double)b.x;
@@ -1061,7 +1090,7 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
                 const uint32_t ni = nfirst + (q - p);
                 S.nodes[ni] = NodeHot{b.x, b.y, b.z, node_word(1u, 0u)};
                 S.ncold[ni] = NodeCold{q, 1u, 0.0f,
-                                       kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue)};
+                                       kTagBody | (q == p ? kTagFirst : 0u) | (body & kTagValue)};
                 S.bnode[q] = ni;
             }
             const double comx = wx / mass, comy = wy / mass;

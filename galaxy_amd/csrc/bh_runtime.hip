@@ -100,7 +100,6 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.val0 = (uint32_t*)take(4ull * n);
     d.val1 = (uint32_t*)take(4ull * n);
     d.sbody = (float4*)take(16ull * n);
-    d.pos_of = (uint32_t*)take(4ull * n);
     d.cbase = (uint32_t*)take(4ull * n);
     d.cparent = (uint32_t*)take(4ull * n);
     d.cdmin = (uint8_t*)take(1ull * n);
@@ -156,6 +155,8 @@ struct SimSet {
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
     size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0;
     uint32_t sort_kpt = 8;
+    uint32_t trav_block = 256;        // traversal workgroup size (64, 128 or 256)
+    uint32_t trav_waves_per_cu = 32;  // persistent traversal wavefronts per CU
     uint32_t traverse_blocks = 0;
 
     ~SimSet() { release(); }
@@ -236,10 +237,12 @@ struct SimSet {
         HIP_OK(hipMemsetAsync(d_ctl, 0, sizeof(uint32_t) * kCtlWords, stream));
         HIP_OK(hipMemsetAsync(d_sflags, 0, sizeof(uint32_t) * 4ull * kRadix * std::max<size_t>(tsort.size(), 1), stream));
         HIP_OK(hipMemsetAsync(d_tflags, 0, sizeof(unsigned long long) * std::max<size_t>(ttopo.size(), 1), stream));
-        // persistent traversal grid: enough 256-thread blocks to fill the device
-        uint32_t waves = (uint32_t)twave.size();
-        uint32_t want = std::max<uint32_t>(1, (uint32_t)dev->cu_count * 8);
-        traverse_blocks = std::max<uint32_t>(1, std::min<uint32_t>(want, div_up(waves, 4)));
+        // persistent traversal grid: trav_waves_per_cu wavefronts per CU, no more
+        // workgroups than there are work items to share
+        const uint32_t waves = (uint32_t)twave.size();
+        const uint32_t per_block = trav_block / 64;
+        const uint32_t want = std::max<uint32_t>(1, (uint32_t)dev->cu_count * trav_waves_per_cu / per_block);
+        traverse_blocks = std::max<uint32_t>(1, std::min<uint32_t>(want, div_up(waves, per_block)));
     }
 
     void upload_desc(hipStream_t stream) {
@@ -306,11 +309,12 @@ struct SimSet {
             HIP_OK(hipGetLastError());
         }
         if (ev_stats || ev_hash)
-            hipLaunchKernelGGL(k_traverse_nodes<true>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
-                               d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, ev_sim);
+            hipLaunchKernelGGL(k_traverse_nodes<true>, dim3(traverse_blocks), dim3(trav_block), 0, s,
+                               d_desc, d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, ev_sim);
         else
-            hipLaunchKernelGGL(k_traverse_nodes<false>, dim3(traverse_blocks), dim3(256), 0, s, d_desc,
-                               d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, 0xffffffffu);
+            hipLaunchKernelGGL(k_traverse_nodes<false>, dim3(traverse_blocks), dim3(trav_block), 0, s,
+                               d_desc, d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash,
+                               0xffffffffu);
         HIP_OK(hipGetLastError());
     }
     void final_kick(hipStream_t s) {
@@ -473,8 +477,12 @@ void read_tree(const SimDesc& d, TreeHost& h, hipStream_t s) {
     HIP_OK(hipMemcpyAsync(h.child.data(), d.cchild, 16ull * h.ncells, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(h.key.data(), d.key0, 4ull * d.n, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(h.val.data(), d.val0, 4ull * d.n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(h.pos_of.data(), d.pos_of, 4ull * d.n, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    // target positions (bh_parallel_tree_positions): the inverse of the sort permutation
+    for (uint32_t p = 0; p < d.n; ++p) {
+        if (h.val[p] >= d.n) throw HipError{"tree has an invalid sort permutation"};
+        h.pos_of[h.val[p]] = p;
+    }
 }
 
 void flat_cell(const TreeHost& h, uint32_t k, GbhGpuCell& c) {
@@ -1140,7 +1148,7 @@ int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]) {
     });
 }
 
-int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]) {
+int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[3], uint32_t first_bad[3]) {
     return guarded([&] {
         if (!dev || !mismatches || !first_bad) {
             set_error("null argument");
@@ -1149,14 +1157,14 @@ int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_b
         hipStream_t s = dev->stream;
         unsigned long long* counts = nullptr;
         uint32_t* first = nullptr;
-        HIP_OK(hipMalloc((void**)&counts, 16));
-        HIP_OK(hipMalloc((void**)&first, 8));
-        HIP_OK(hipMemsetAsync(counts, 0, 16, s));
-        HIP_OK(hipMemsetAsync(first, 0xff, 8, s));
+        HIP_OK(hipMalloc((void**)&counts, 24));
+        HIP_OK(hipMalloc((void**)&first, 12));
+        HIP_OK(hipMemsetAsync(counts, 0, 24, s));
+        HIP_OK(hipMemsetAsync(first, 0xff, 12, s));
         hipLaunchKernelGGL(k_check_rounding, dim3(4096), dim3(256), 0, s, counts, first);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(mismatches, counts, 16, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(first_bad, first, 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(mismatches, counts, 24, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(first_bad, first, 12, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         HIP_OK(hipFree(counts));
         HIP_OK(hipFree(first));

@@ -191,10 +191,12 @@ typedef struct {
     double dt;
 } gbh_sim_params;
 
+/* Launch-shape choices; none of them changes any result.  A zero field keeps
+ * its default (2048, 256, 32). */
 typedef struct {
     uint32_t sort_tile;      /* keys per sort workgroup: 1024 or 2048 */
     uint32_t traverse_block; /* traversal workgroup size: 64, 128, 256 */
-    uint32_t traverse_waves_per_cu; /* persistent traversal occupancy target */
+    uint32_t traverse_waves_per_cu; /* persistent traversal wavefronts per CU, 1..32 */
     uint32_t use_graph;      /* capture each step in a HIP graph */
 } gbh_tuning;
 
@@ -262,9 +264,11 @@ int gbh_diag_direct(gbh_state* st, const gbh_config* cfg, uint32_t count, const 
                     double* ax, double* ay);
 int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]);
 /* Exhaustive device check (no reference counterpart) that the traversal's f32
- * square root and reciprocal are correctly rounded for every positive f32:
- * mismatches[0] = sqrt, [1] = reciprocal; first_bad = smallest failing pattern. */
-int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[2], uint32_t first_bad[2]);
+ * square root and reciprocals are correctly rounded: mismatches[0] = sqrt and
+ * [1] = reciprocal over every non-negative f32, [2] = the reciprocal of the
+ * guard-free interaction over every f32 in [2^-126, 2^126); first_bad = smallest
+ * failing pattern of each. */
+int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[3], uint32_t first_bad[3]);
 
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel
  * (k_copy_calibration); GPU time in *ms.  Used to calibrate hardware traffic

@@ -38,12 +38,19 @@ def mixed_sims():
     ]
 
 
-@pytest.mark.parametrize("use_graph,sort_tile", [(True, 2048), (False, 1024)])
-def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile):
+@pytest.mark.parametrize("use_graph,sort_tile,block,waves,bucket", [
+    (True, 2048, 256, 32, None), (False, 1024, 256, 32, None), (True, 1024, 64, 16, 8),
+    (True, 2048, 128, 24, 16)])
+def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile, block, waves, bucket):
+    """Every launch shape the autotuner can pick, and other leaf sizes, give the
+    oracle's trajectory for every simulation of the batch."""
     sims = mixed_sims()
-    print(f"batch graph={use_graph} sort_tile={sort_tile}", flush=True)
+    if bucket is not None:
+        sims = [(s, Config(c.theta, c.softening_kpc, c.g, bucket, c.max_depth), dt) for s, c, dt in sims]
+    print(f"batch graph={use_graph} sort_tile={sort_tile} block={block} waves={waves} bucket={bucket}",
+          flush=True)
     b = Batch(gpu, [s for s, _, _ in sims], [c for _, c, _ in sims], [dt for _, _, dt in sims],
-              use_graph=use_graph, sort_tile=sort_tile)
+              use_graph=use_graph, sort_tile=sort_tile, traverse_block=block, traverse_waves_per_cu=waves)
     b.step(2)
     b.step(1)
     for i, (s0, cfg, dt) in enumerate(sims):
@@ -98,3 +105,27 @@ def test_server_budget_admission_and_staggered_arrivals(oracle, gpu):
         assert np.array_equal(srv.take(rid)[0].view(np.uint32), rs.view(np.uint32))
     with pytest.raises(Exception, match="HBM budget"):
         Server(gpu, hbm_budget_bytes=1 << 20, steps_per_round=10).submit(*sims[1], 10)
+
+
+def test_tuning_rejects_unsupported_shapes(gpu):
+    sims = mixed_sims()[:1]
+    for kw, msg in [(dict(sort_tile=512), "sort_tile"), (dict(traverse_block=96), "traverse_block"),
+                    (dict(traverse_waves_per_cu=33), "traverse_waves_per_cu")]:
+        with pytest.raises(Exception, match=msg):
+            Batch(gpu, [sims[0][0]], sims[0][1], sims[0][2], **kw)
+
+
+def test_autotune_picks_a_measured_choice_and_caches_it(gpu, tmp_path):
+    from galaxy_amd import autotune
+
+    states = [ics.rotating_disc(n, 2000 + n) for n in (900, 3000, 12000)]
+    cfg = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    cache = str(tmp_path / "tune.json")
+    print("autotune small ensemble", flush=True)
+    r = autotune.tune(gpu, states, cfg, 1e-3, cache=cache, steps=1)
+    assert not r["cached"]
+    measured = {tuple(sorted(c.items())): t for c, t in r["measured"]}
+    best = tuple(sorted(autotune.asdict(r["choice"]).items()))
+    assert measured[best] == min(measured.values())
+    again = autotune.tune(gpu, states, cfg, 1e-3, cache=cache, steps=1)
+    assert again["cached"] and again["choice"] == r["choice"]
