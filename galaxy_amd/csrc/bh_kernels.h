@@ -72,7 +72,7 @@ struct __attribute__((aligned(32))) CellT {
 // thresholds do not follow the preorder (general mode).
 struct __attribute__((aligned(16))) NodeHot {
     float x, y, m;
-    uint32_t w;  // (subtree size << kSizeShift) | (code << 2)
+    uint32_t w;  // ((subtree size - 1) << kSizeShift) | (code << 2)
 };
 struct __attribute__((aligned(16))) NodeCold {
     uint32_t start, count;
@@ -83,8 +83,10 @@ constexpr uint32_t kSizeShift = 6;
 constexpr uint32_t kCodes = 16;
 constexpr uint32_t kCodeField = (kCodes - 1u) << 2;          // byte offset into the code table
 constexpr uint32_t kMaxNodes = 1u << (32 - kSizeShift);      // subtree sizes fit the hot word
+// The word holds size - 1, so a walk advances by at least one node per step
+// whatever the word holds.
 __host__ __device__ constexpr uint32_t node_word(uint32_t size, uint32_t code) {
-    return (size << kSizeShift) | (code << 2);
+    return ((size - 1u) << kSizeShift) | (code << 2);
 }
 constexpr uint32_t kTagBody = 1u << 31;
 constexpr uint32_t kTagFirst = 1u << 30;  // first body of its leaf
@@ -134,6 +136,7 @@ struct __attribute__((aligned(16))) SimDesc {
     uint8_t* ccount;   // cells in the column
     // cells (preorder)
     CellT* cells;
+    uint32_t* cend;    // end of each cell's range (written by k_topo_count)
     float4* cgeom;     // f32 (cx, cy, half, 0)
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
@@ -808,7 +811,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
 // the keys: per-wavefront ballot masks of "dmin <= d" in LDS, per-wavefront
 // first/last boundaries within the tile, and per-tile summaries over the other
 // tiles of the simulation (k_topo_pre, k_topo_scan).
-constexpr uint32_t kTopoWaves = kTopoTile / 64;
+constexpr uint32_t kTopoWaves = kTopoTile / 64;          // 64-position chunks of a tile
+constexpr uint32_t kTopoThreads = 1024;                   // threads per topology workgroup
+constexpr uint32_t kTopoPer = kTopoTile / kTopoThreads;   // positions per thread (strided)
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
 constexpr uint32_t kNone = 0xffffffffu;
 
@@ -821,32 +826,39 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 // Per topology tile: dmin of every position (cdmin) and, per depth d, the first
 // boundary position in the tile and one past the last (tsum[t][d], tsum[t][17+d];
 // kNone / 0 when the tile has none).
-__global__ __launch_bounds__(kTopoTile) void k_topo_pre(const SimDesc* __restrict__ sims,
-                                                        const TileRef* __restrict__ ttiles,
-                                                        uint32_t* __restrict__ tsum) {
+__global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
+                                                           const TileRef* __restrict__ ttiles,
+                                                           uint32_t* __restrict__ tsum) {
     __shared__ uint32_t s_first[kLevels], s_last[kLevels];
     const uint32_t t = blockIdx.x;
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
-    const uint32_t p = tr.base + threadIdx.x;
-    const bool valid = p < n;
     if (threadIdx.x < kLevels) {
         s_first[threadIdx.x] = kNone;
         s_last[threadIdx.x] = 0u;
     }
-    uint32_t dmin = 17u;
-    if (valid) {
-        dmin = dmin_of(S.key0, p, S.key0[p]);
-        S.cdmin[p] = (uint8_t)dmin;
+    uint32_t dmin[kTopoPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+        dmin[j] = 17u;
+        if (p < n) {
+            dmin[j] = dmin_of(S.key0, p, S.key0[p]);
+            S.cdmin[p] = (uint8_t)dmin[j];
+        }
     }
     __syncthreads();
-    const uint32_t wbase = tr.base + (threadIdx.x & ~63u);
-    for (uint32_t d = 0; d < kLevels; ++d) {
-        const unsigned long long m = __ballot(valid && dmin <= d);
-        if (m && lane_id() == 0) {
-            atomicMin(&s_first[d], wbase + (uint32_t)__builtin_ctzll(m));
-            atomicMax(&s_last[d], wbase + 64u - (uint32_t)__builtin_clzll(m));
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+        const uint32_t wbase = p - lane_id();
+        for (uint32_t d = 0; d < kLevels; ++d) {
+            const unsigned long long m = __ballot(p < n && dmin[j] <= d);
+            if (m && lane_id() == 0) {
+                atomicMin(&s_first[d], wbase + (uint32_t)__builtin_ctzll(m));
+                atomicMax(&s_last[d], wbase + 64u - (uint32_t)__builtin_clzll(m));
+            }
         }
     }
     __syncthreads();
@@ -913,12 +925,16 @@ struct TileBounds {
     uint32_t after[kTopoWaves][kLevels];           // first boundary in later waves (kNone: none)
     uint32_t pre[kLevels], post[kLevels];          // tile summaries of the other tiles
 
-    // Every thread of the tile calls this (block-wide barriers).
-    __device__ void build(uint32_t base, bool valid, uint32_t dmin, const uint32_t* __restrict__ ps) {
-        const uint32_t w = threadIdx.x >> 6;
-        for (uint32_t d = 0; d < kLevels; ++d) {
-            const unsigned long long m = __ballot(valid && dmin <= d);
-            if (lane_id() == 0) mask[w][d] = m;
+    // Every thread of the workgroup calls this (block-wide barriers), after a
+    // barrier that publishes dmin[] (LDS, one byte per position of the tile).
+    __device__ void build(uint32_t base, uint32_t n, const uint8_t* dmin, const uint32_t* __restrict__ ps) {
+        for (uint32_t c = threadIdx.x >> 6; c < kTopoWaves; c += blockDim.x >> 6) {
+            const uint32_t o = c * 64u + lane_id();
+            const uint32_t dm = base + o < n ? dmin[o] : 17u;
+            for (uint32_t d = 0; d < kLevels; ++d) {
+                const unsigned long long m = __ballot(dm <= d);
+                if (lane_id() == 0) mask[c][d] = m;
+            }
         }
         if (threadIdx.x < kLevels) {
             pre[threadIdx.x] = ps[threadIdx.x];
@@ -961,19 +977,23 @@ struct TileBounds {
     }
 };
 
-__global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __restrict__ sims,
-                                                          const TileRef* __restrict__ ttiles,
-                                                          uint32_t* __restrict__ ctl,
-                                                          unsigned long long* __restrict__ tflags,
-                                                          const uint32_t* __restrict__ psum) {
-    __shared__ uint32_t s_scan[kTopoTile / 64];
+__global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __restrict__ sims,
+                                                             const TileRef* __restrict__ ttiles,
+                                                             uint32_t* __restrict__ ctl,
+                                                             unsigned long long* __restrict__ tflags,
+                                                             const uint32_t* __restrict__ psum) {
+    // Each thread handles positions base + j * kTopoThreads + threadIdx.x, j < kTopoPer,
+    // so a wavefront covers 64 consecutive positions in every slice j and the
+    // per-block latency chain (ticket, tile, look-back) is paid once per 1024
+    // positions by 256 threads.  Per-position state lives in LDS between phases.
+    __shared__ uint32_t s_scan[kTopoThreads / 64];
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
     __shared__ TileBounds tb;
-    // the tile's keys, sorted bodies and body indices, staged once with coalesced
-    // loads: existence, leaf-depth and leaf-body reads inside the tile come from LDS
-    __shared__ uint32_t s_key[kTopoTile], s_val[kTopoTile];
-    __shared__ float4 s_body[kTopoTile];
+    __shared__ uint32_t s_key[kTopoTile];   // keys of the tile
+    __shared__ uint32_t s_pl[kTopoTile];    // start of the parent group of a column head
+    __shared__ uint32_t s_mine[kTopoTile];  // exclusive (columns << 16 | cells) prefix in the tile
+    __shared__ uint8_t s_dmin[kTopoTile], s_count[kTopoTile];
     if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
     __syncthreads();
     const uint32_t t = s_ticket;
@@ -981,25 +1001,31 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
-    const uint32_t p = tr.base + threadIdx.x;
-    const bool valid = p < n;
-    const uint32_t kp = valid ? K[p] : 0u;
-    uint32_t dmin = valid ? (uint32_t)S.cdmin[p] : 17u;
-    if (valid) {
-        s_key[threadIdx.x] = kp;
-        s_val[threadIdx.x] = S.val0[p];
-        s_body[threadIdx.x] = S.sbody[p];
+    const uint32_t base = tr.base;
+    const uint32_t tile_end = min(base + kTopoTile, n);
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t o = j * kTopoThreads + threadIdx.x;
+        const uint32_t p = base + o;
+        s_key[o] = p < n ? K[p] : 0u;
+        s_dmin[o] = p < n ? S.cdmin[p] : (uint8_t)17u;
     }
-    const uint32_t tile_end = min(tr.base + kTopoTile, n);
-    auto key_at = [&](uint32_t q) { return q - tr.base < tile_end - tr.base ? s_key[q - tr.base] : K[q]; };
-    tb.build(tr.base, valid, dmin, psum + (size_t)t * kTopoSum);  // its barriers publish the staging
-    uint32_t count = 0, dleaf = 0, pl = 0;
-    if (valid) {
-        if (dmin <= maxd) {
+    __syncthreads();
+    tb.build(base, n, s_dmin, psum + (size_t)t * kTopoSum);
+    auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
+    // columns: existence from the parent group's size, leaf depth from the key at p + bucket
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t o = j * kTopoThreads + threadIdx.x;
+        const uint32_t p = base + o;
+        uint32_t count = 0, pl = 0;
+        const uint32_t dmin = s_dmin[o];
+        if (p < n && dmin <= maxd) {
+            const uint32_t kp = s_key[o];
             bool exists = true;
             if (dmin >= 1) {
                 const uint32_t dd = dmin - 1;
-                pl = tb.prev(tr.base, p, dd);
+                pl = tb.prev(base, p, dd);
                 const uint32_t q = pl + bucket;
                 exists = (q < n) && same_prefix(key_at(q), kp, dd);
             }
@@ -1007,16 +1033,24 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
                 const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
                 int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
                 if (dl < (int)dmin) dl = (int)dmin;
-                dleaf = (uint32_t)dl;
-                count = dleaf - dmin + 1;
+                count = (uint32_t)dl - dmin + 1;
             }
         }
+        s_count[o] = (uint8_t)count;
+        s_pl[o] = pl;
     }
-    // cells (low 16 bits) and columns (high 16 bits) of the tile in one scan:
-    // a tile holds at most 17 * 1024 cells
-    uint32_t total;
-    const uint32_t packed = count | ((count > 0 ? 1u : 0u) << 16);
-    const uint32_t mine = block_excl_sum<kTopoTile>(packed, s_scan, total);
+    // cells (low 16 bits) and columns (high 16 bits) in position order, slice by
+    // slice: a tile holds at most 17 * 1024 cells
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t o = j * kTopoThreads + threadIdx.x;
+        const uint32_t count = s_count[o];
+        uint32_t part;
+        const uint32_t mine = block_excl_sum<kTopoThreads>(count | ((count > 0 ? 1u : 0u) << 16), s_scan, part);
+        s_mine[o] = total + mine;
+        total += part;
+    }
     if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
         unsigned long long* f = tflags + t;
         const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
@@ -1035,7 +1069,7 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
         }
         s_excl = (uint32_t)(excl & 0x7fffffffull);
         s_excl_cols = (uint32_t)(excl >> 31);
-        if (threadIdx.x == 0 && tr.base + kTopoTile >= n) {
+        if (threadIdx.x == 0 && base + kTopoTile >= n) {
             const uint32_t ncells = s_excl + (total & 0xffffu);
             const uint32_t ncols = s_excl_cols + (total >> 16);
             S.meta->ncells = ncells;
@@ -1043,76 +1077,89 @@ __global__ __launch_bounds__(kTopoTile, 8) void k_topo_count(const SimDesc* __re
         }
     }
     __syncthreads();
-    if (!valid) return;
-    const uint32_t base = s_excl + (mine & 0xffffu);
-    // node index of the column's first cell: internal cells and bodies before p
-    const uint32_t nb = base - (s_excl_cols + (mine >> 16)) + p;
-    S.cbase[p] = base;
-    S.nbase[p] = nb;
-    S.ccount[p] = (uint8_t)count;
-    S.cparent[p] = pl;
-    if (count == 0) return;
-    if (base + count > S.cell_cap) {
-        atomicOr(&S.meta->err, kErrCellCapacity);
-        return;
-    }
-    // geometry at dmin along the key's quadrant path (child_bounds, flat.rs:96-103)
     const SimMeta* M = S.meta;
-    double cx = M->root[0], cy = M->root[1], h = M->root[2];
-    for (uint32_t l = 0; l < dmin; ++l) {
-        const uint32_t q = (kp >> (30u - 2u * l)) & 3u;
-        h = h * 0.5;
-        cx = cx + ((q & 1u) ? h : -h);
-        cy = cy + ((q & 2u) ? h : -h);
-    }
-    for (uint32_t d = dmin; d <= dleaf; ++d) {
-        const uint32_t idx = base + (d - dmin);
-        const uint32_t e = tb.next(tr.base, p, d);
-        const bool leaf = (d == dleaf);
-        CellT c;
-        c.start = p;
-        c.end = e;
-        c.skip = 0u;
-        c.info = d | (leaf ? kInfoLeaf : 0u);
-        c.T = M->T[d];
-        uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
-        if (leaf) {
-            double mass = 0.0, wx = 0.0, wy = 0.0;
-            const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
-            for (uint32_t q = p; q < e; ++q) {
-                const bool here = q < tile_end;
-                const float4 b = here ? s_body[q - tr.base] : S.sbody[q];
-                const uint32_t body = here ? s_val[q - tr.base] : S.val0[q];
-                const double bm = b.z;
-                mass = mass + bm;
-                wx = wx + bm * (double)b.x;
-                wy = wy + bm * (double)b.y;
-                const uint32_t ni = nfirst + (q - p);
-                S.nodes[ni] = NodeHot{b.x, b.y, b.z, node_word(1u, 0u)};
-                S.ncold[ni] = NodeCold{q, 1u, 0.0f,
-                                       kTagBody | (q == p ? kTagFirst : 0u) | (body & kTagValue)};
-                S.bnode[q] = ni;
-            }
-            const double comx = wx / mass, comy = wy / mass;
-            S.cagg[idx] = Agg{mass, comx, comy, 0.0};
-            c.comx = (float)comx;
-            c.comy = (float)comy;
-            c.mass = (float)mass;
-        } else {
-            const uint32_t q0 = (kp >> (30u - 2u * d)) & 3u;
-            (&ch.x)[q0] = idx + 1u;
-            c.comx = 0.f;
-            c.comy = 0.f;
-            c.mass = 0.f;
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t o = j * kTopoThreads + threadIdx.x;
+        const uint32_t p = base + o;
+        if (p >= n) break;
+        const uint32_t mine = s_mine[o];
+        const uint32_t count = s_count[o];
+        const uint32_t dmin = s_dmin[o];
+        const uint32_t kp = s_key[o];
+        const uint32_t cbase = s_excl + (mine & 0xffffu);
+        // node index of the column's first cell: internal cells and bodies before p
+        const uint32_t nb = cbase - (s_excl_cols + (mine >> 16)) + p;
+        S.cbase[p] = cbase;
+        S.nbase[p] = nb;
+        S.ccount[p] = (uint8_t)count;
+        S.cparent[p] = s_pl[o];
+        if (count == 0) continue;
+        if (cbase + count > S.cell_cap) {
+            atomicOr(&S.meta->err, kErrCellCapacity);
+            continue;
         }
-        S.cells[idx] = c;
-        S.cchild[idx] = ch;
-        S.cgeom[idx] = make_float4((float)cx, (float)cy, (float)h, 0.0f);
-        if (!leaf) {
-            const uint32_t q = (kp >> (30u - 2u * d)) & 3u;
+        const uint32_t dleaf = dmin + count - 1u;
+        // geometry at dmin along the key's quadrant path (child_bounds, flat.rs:96-103)
+        double cx = M->root[0], cy = M->root[1], h = M->root[2];
+        for (uint32_t l = 0; l < dmin; ++l) {
+            const uint32_t q = (kp >> (30u - 2u * l)) & 3u;
             h = h * 0.5;
             cx = cx + ((q & 1u) ? h : -h);
             cy = cy + ((q & 2u) ? h : -h);
+        }
+        for (uint32_t d = dmin; d <= dleaf; ++d) {
+            const uint32_t idx = cbase + (d - dmin);
+            const uint32_t e = tb.next(base, p, d);
+            const bool leaf = (d == dleaf);
+            CellT c;
+            c.start = p;
+            c.end = e;
+            c.skip = 0u;
+            c.info = d | (leaf ? kInfoLeaf : 0u);
+            c.T = M->T[d];
+            uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
+            if (leaf) {
+                double mass = 0.0, wx = 0.0, wy = 0.0;
+                const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
+                for (uint32_t q = p; q < e; ++q) {
+                    const float4 bq = S.sbody[q];
+                    const double bm = bq.z;
+                    mass = mass + bm;
+                    wx = wx + bm * (double)bq.x;
+                    wy = wy + bm * (double)bq.y;
+                    const uint32_t ni = nfirst + (q - p);
+                    S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u)};
+                    S.ncold[ni] = NodeCold{q, 1u, 0.0f,
+                                           kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue)};
+                    S.bnode[q] = ni;
+                }
+                const double comx = wx / mass, comy = wy / mass;
+                S.cagg[idx] = Agg{mass, comx, comy, 0.0};
+                c.comx = (float)comx;
+                c.comy = (float)comy;
+                c.mass = (float)mass;
+            } else {
+                const uint32_t q0 = (kp >> (30u - 2u * d)) & 3u;  // first child: same column, depth d+1
+                ch.x = q0 == 0u ? idx + 1u : kNoChild;
+                ch.y = q0 == 1u ? idx + 1u : kNoChild;
+                ch.z = q0 == 2u ? idx + 1u : kNoChild;
+                ch.w = q0 == 3u ? idx + 1u : kNoChild;
+                c.comx = 0.f;
+                c.comy = 0.f;
+                c.mass = 0.f;
+            }
+            // internal cells' records are written whole by k_topo_link once their
+            // aggregates are known; here only their range end and first child
+            if (leaf) S.cells[idx] = c;
+            S.cend[idx] = e;
+            S.cchild[idx] = ch;
+            S.cgeom[idx] = make_float4((float)cx, (float)cy, (float)h, 0.0f);
+            if (!leaf) {
+                const uint32_t q = (kp >> (30u - 2u * d)) & 3u;
+                h = h * 0.5;
+                cx = cx + ((q & 1u) ? h : -h);
+                cy = cy + ((q & 2u) ? h : -h);
+            }
         }
     }
 }
@@ -1139,9 +1186,12 @@ __device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, ui
 // bottom-up aggregates of every internal cell whose range lies inside this tile.
 // Internal cells that cross a tile boundary go to the simulation's span list.
 // The tile's cells are a contiguous preorder range [cb0, cb1); when it fits
-// kLinkCells, children links and f64 aggregates are kept in LDS, so each depth
-// round of the bottom-up pass is an LDS round trip, and only depths present in
-// the tile are visited.  Larger tiles use the same rounds through global memory.
+// kLinkCells, children links, node words, range ends and f64 aggregates are kept
+// in LDS, so each depth round of the bottom-up pass is an LDS round trip, and
+// only depths present in the tile are visited.  Every internal cell's records
+// (cell, hot node, child links, aggregate) are written whole, once, in its
+// round: no partial updates of lines written by another kernel.  Larger tiles
+// use the same rounds through global memory.
 // ----------------------------------------------------------------------------
 constexpr uint32_t kLinkCells = 1536;
 
@@ -1149,6 +1199,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                                                          const TileRef* __restrict__ ttiles) {
     __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];
     __shared__ uint32_t s_ch[kLinkCells][4];
+    __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];
     __shared__ uint32_t s_cb0, s_cb1, s_dlo, s_dhi;
     const TileRef tr = ttiles[blockIdx.x];
     const SimDesc& S = sim_at(sims, tr.sim);
@@ -1187,23 +1238,27 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             s_ch[i][0] = s_ch[i][1] = s_ch[i][2] = s_ch[i][3] = kNoChild;
     __syncthreads();
     uint32_t inside = 0;  // bit k: cell base + k is internal and lies inside the tile
+    const uint32_t kp = ok ? S.key0[p] : 0u;
     if (ok) {
-        const uint32_t kp = S.key0[p];
         for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t e = S.cells[base + k].end;
+            const uint32_t idx = base + k;
+            const uint32_t e = S.cend[idx];
             const uint32_t d = dmin + k;
-            if (k + 1u < count) {  // internal: its node (aggregates follow below)
+            if (k + 1u < count) {  // internal: cold node now, the rest in its round
                 const uint32_t ni = nb + k;
                 const uint32_t skip = e < n ? S.nbase[e] : nnodes;
-                S.nodes[ni].w = node_word(skip - ni, d);  // internal depths are 0..15
                 S.ncold[ni] = NodeCold{p, e - p, S.meta->T[d], d};
                 if (e <= tile_end) inside |= 1u << k;
-                if (in_lds) s_ch[base + k - cb0][(kp >> (30u - 2u * d)) & 3u] = base + k + 1u;
+                if (in_lds) {
+                    s_w[idx - cb0] = node_word(skip - ni, d);  // internal depths are 0..15
+                    s_e[idx - cb0] = e;
+                    s_ch[idx - cb0][(kp >> (30u - 2u * d)) & 3u] = idx + 1u;
+                }
             } else if (in_lds) {  // the leaf's aggregate (k_topo_count)
-                const Agg a = S.cagg[base + k];
-                s_m[base + k - cb0] = a.m;
-                s_x[base + k - cb0] = a.x;
-                s_y[base + k - cb0] = a.y;
+                const Agg a = S.cagg[idx];
+                s_m[idx - cb0] = a.m;
+                s_x[idx - cb0] = a.x;
+                s_y[idx - cb0] = a.y;
             }
         }
         if (count > 1u) {
@@ -1215,8 +1270,8 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             const uint32_t parent = S.cbase[pl] + (dmin - 1u - S.cdmin[pl]);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
             if (S.cdmin[pl] <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
-                (&S.cchild[parent].x)[q] = base;
                 if (in_lds && parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = base;
+                else (&S.cchild[parent].x)[q] = base;  // a parent outside the tile's LDS range
             } else {
                 atomicOr(&S.meta->err, kErrCellCapacity);
             }
@@ -1228,13 +1283,25 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
         const uint32_t k = (uint32_t)d - dmin;
         if (count > 1u && (uint32_t)d >= dmin && k + 1u < count) {
             const uint32_t idx = base + k;
+            const uint32_t ni = nb + k;
+            uint32_t e, w;
+            if (in_lds) {
+                e = s_e[idx - cb0];
+                w = s_w[idx - cb0];
+            } else {
+                e = S.cend[idx];
+                w = node_word((e < n ? S.nbase[e] : nnodes) - ni, (uint32_t)d);
+            }
+            const float T = S.meta->T[d];
             if (inside & (1u << k)) {
                 Agg a;
                 if (in_lds) {
                     double mass = 0.0, wx = 0.0, wy = 0.0;
+                    uint4 ch;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t c = s_ch[idx - cb0][q];
+                        (&ch.x)[q] = c;
                         if (c != kNoChild) {
                             const double cm = s_m[c - cb0];
                             mass = mass + cm;
@@ -1246,24 +1313,31 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                     s_m[idx - cb0] = a.m;
                     s_x[idx - cb0] = a.x;
                     s_y[idx - cb0] = a.y;
+                    S.cchild[idx] = ch;
                 } else {
                     a = combine_children(S.cagg, S.cchild[idx]);
                 }
                 S.cagg[idx] = a;
-                S.cells[idx].comx = (float)a.x;
-                S.cells[idx].comy = (float)a.y;
-                S.cells[idx].mass = (float)a.m;
-                NodeHot* nd = &S.nodes[nb + k];
-                nd->x = (float)a.x;
-                nd->y = (float)a.y;
-                nd->m = (float)a.m;
+                S.cells[idx] = CellT{(float)a.x, (float)a.y, (float)a.m, T, p, e, 0u, (uint32_t)d};
+                S.nodes[ni] = NodeHot{(float)a.x, (float)a.y, (float)a.m, w};
             } else {
+                uint32_t info = (uint32_t)d;
                 const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
                 if (slot < S.span_cap) {
                     S.span[slot] = idx;
-                    S.cells[idx].info |= (slot + 1u) << kInfoSpanShift;
+                    info |= (slot + 1u) << kInfoSpanShift;
                 } else {
                     atomicOr(&S.meta->err, kErrSpanCapacity);
+                }
+                // aggregates follow in k_agg_top; the cell's other fields are final
+                S.cells[idx] = CellT{0.f, 0.f, 0.f, T, p, e, 0u, info};
+                S.nodes[ni] = NodeHot{0.f, 0.f, 0.f, w};
+                if (in_lds) {  // children inside the tile; others were written directly
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t c = s_ch[idx - cb0][q];
+                        if (c != kNoChild) (&S.cchild[idx].x)[q] = c;
+                    }
                 }
             }
         }
@@ -1428,9 +1502,10 @@ struct WalkEvidence {
 // simulation's positions lie in a box of half-width < 1e18 and
 // 0 < eps2 <= FLT_MAX), where point_mass_pos equals point_mass_d bit for bit.
 // The walk is software-pipelined: the next node's gather is issued before the
-// interaction of the current one is evaluated.  The cursor never passes nn
-// (a subtree ends inside the stream), and the node arrays hold at least nn + 1
-// records, so the gather at nn stays in bounds and its value is not used.
+// interaction of the current one is evaluated.  Whatever the node words hold,
+// every step advances the cursor by at least one node and the early gather is
+// clamped to index nn (the node arrays hold at least nn + 1 records, and the
+// value read there is never used), so a walk always ends and stays in bounds.
 // `tab` is the byte address of the wavefront's code table in LDS.
 template <bool kEvidence, bool kGeneral, bool kFast>
 struct NodeWalk {
@@ -1450,7 +1525,7 @@ struct NodeWalk {
         const float dd = dx * dx;
         const float ee = dy * dy;
         const float d2 = dd + ee;
-        const uint32_t size = w >> kSizeShift;
+        const uint32_t size_m1 = w >> kSizeShift;  // subtree [c, c + size_m1]
         bool inside;
         float T;
         U4v l;
@@ -1459,12 +1534,12 @@ struct NodeWalk {
             inside = (p - l.x) < l.y;
             T = __uint_as_float(l.z);
         } else {
-            inside = (self - c) < size;
+            inside = (self - c) <= size_m1;
             T = *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
         }
         const bool acc = !inside && d2 >= T;
-        const uint32_t next = c + (acc ? size : 1u);  // size >= 1: the walk ends
-        gn = nh[next];
+        const uint32_t next = c + 1u + (acc ? size_m1 : 0u);  // at least one node per step
+        gn = nh[min(next, nn)];
         if (kEvidence) {
             if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++ev.visited;  // a cell (a leaf: its first body)
         }

@@ -105,6 +105,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cdmin = (uint8_t*)take(1ull * n);
     d.ccount = (uint8_t*)take(1ull * n);
     d.cells = (CellT*)take(sizeof(CellT) * cap);
+    d.cend = (uint32_t*)take(4ull * cap);
     d.cgeom = (float4*)take(16ull * cap);
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
@@ -279,13 +280,13 @@ struct SimSet {
         }
     }
     void topo(hipStream_t s) {
-        hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
+        hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
                            d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size()), dim3(1024), 0, s, d_desc, d_tsum,
                            d_psum);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_count, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
+        hipLaunchKernelGGL(k_topo_count, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
                            d_ttopo, d_ctl, d_tflags, d_psum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_link, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
