@@ -84,7 +84,42 @@ def plain_run(gpu, steps, warmup):
                 max_depth=maxd)
 
 
-def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None):
+def large_run(gpu, steps, warmup):
+    """One simulation beyond the reference cap: binary disc encounter, N = 1,048,576,
+    theta 0.6, through the same synchronous drop-in calls with the beyond-cap tree."""
+    from galaxy_amd import ParallelTreeRuntime, ics
+
+    w = ics.config2()
+    rt = ParallelTreeRuntime(gpu)
+    ev = gpu.create_evolving_state(w.states[0])
+    tree = rt.create_tree(gpu, ev, large=True)
+    cfg, dt = w.config, w.dt
+    rt.rebuild(gpu, ev, tree, cfg)
+    rt.force(gpu, ev, tree, cfg)
+
+    def step():
+        gpu.kick_drift(ev, dt)
+        rt.rebuild(gpu, ev, tree, cfg)
+        rt.force(gpu, ev, tree, cfg)
+        gpu.final_kick(ev, dt)
+
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    force_ms = []
+    for _ in range(steps):
+        step()
+        force_ms.append(rt.last_force_ms(tree))
+    el = time.perf_counter() - t0
+    n = len(w.states[0])
+    out = dict(n=n, steps=steps, seconds=el, body_steps_per_s=n * steps / el, ms_per_step=1e3 * el / steps,
+               traversal_ms=float(np.mean(force_ms)), theta=cfg.theta)
+    tree.close()
+    ev.close()
+    return out
+
+
+def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choice=None):
     from dataclasses import asdict, replace
 
     from galaxy_amd import autotune, ics
@@ -92,7 +127,9 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None):
 
     w = ics.config3(count=count)
     tuning = None
-    choice = autotune.Choice()
+    if choice is not None:
+        tune = False
+    choice = choice or autotune.Choice()
     if tune:
         log("  autotuning leaf size / sort tile / traversal launch shape ...")
         tuning = autotune.tune(gpu, w.states, w.config, w.dt, cache=tune_cache, steps=5, log=log)
@@ -201,9 +238,18 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--no-tune", action="store_true", help="batched leg at the default launch shape and leaf size 4")
     ap.add_argument("--tune-cache", default=None, help="JSON cache of autotuner results (default: none, tune every run)")
-    ap.add_argument("--only", choices=["plain", "batched", "copy"], default=None,
+    ap.add_argument("--choice", default=None,
+                    help="fixed batched choice instead of tuning, e.g. bucket=4,sort_tile=2048,"
+                         "traverse_block=256,traverse_waves_per_cu=16")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1,048,576-body single simulation")
+    ap.add_argument("--only", choices=["plain", "batched", "large", "copy"], default=None,
                     help="run one leg only (for profiling); prints that leg's result")
     args = ap.parse_args()
+    choice = None
+    if args.choice:
+        from galaxy_amd.autotune import Choice
+
+        choice = Choice(**{k: int(v) for k, v in (kv.split("=") for kv in args.choice.split(","))})
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the simulations of this workload are independent and "
               f"run on a single MI355X; --gpus {args.gpus} is not supported", file=sys.stderr)
@@ -229,16 +275,25 @@ def main():
     if args.only == "plain":
         print(json.dumps(plain_run(gpu, args.steps, args.warmup)), flush=True)
         return
+    if args.only == "large":
+        print(json.dumps(large_run(gpu, args.steps, args.warmup)), flush=True)
+        return
     if args.only == "batched":
         print(json.dumps(batched_run(gpu, args.steps, args.warmup, tune=not args.no_tune,
-                                     tune_cache=args.tune_cache)), flush=True)
+                                     tune_cache=args.tune_cache, choice=choice)), flush=True)
         return
     log("plain drop-in, cold collapse N=65536 ...")
     plain = plain_run(gpu, args.steps, args.warmup)
     log(f"  plain {plain['body_steps_per_s']:.3e} body-steps/s, {plain['ms_per_step']:.3f} ms/step")
     log("batched ensemble, 256 discs ...")
-    batched = batched_run(gpu, args.steps, args.warmup, tune=not args.no_tune, tune_cache=args.tune_cache)
+    batched = batched_run(gpu, args.steps, args.warmup, tune=not args.no_tune, tune_cache=args.tune_cache,
+                          choice=choice)
     log(f"  batched {batched['body_steps_per_s']:.3e} body-steps/s, {batched['ms_per_step']:.3f} ms/step")
+    large = None
+    if not args.no_large:
+        log("large single simulation, binary disc N=1048576 ...")
+        large = large_run(gpu, args.steps, args.warmup)
+        log(f"  large {large['body_steps_per_s']:.3e} body-steps/s, {large['ms_per_step']:.3f} ms/step")
     serving = None
     if not args.no_serving:
         log("serving sample ...")
@@ -281,6 +336,7 @@ def main():
         "plain_ms_per_step": plain["ms_per_step"],
         "batched_vs_plain": batched["body_steps_per_s"] / plain["body_steps_per_s"],
         "serving": serving,
+        "large_single": large,
         "phase_ms": batched["phase_ms"],
         "roofline": {
             "bound": "mfma",

@@ -105,6 +105,7 @@ def lib():
         "gbh_diag_energy": ([vp, dbl, dbl, vp], i32),
         "gbh_copy_calibration": ([vp, u64, vp], i32),
         "gbh_sparse_layout_bytes": ([u32], u64),
+        "gbh_fnv_mix64": ([u64, vp, u64, vp], i32),
         "gbh_traverse_packed": ([vp, vp, vp, vp, vp, vp, vp], i32),
         "gbh_force_packed": ([vp, vp, vp, vp, vp, vp], i32),
         "gbh_batch_create": ([vp, u32, vp, vp, vp, vp, vp], i32),

@@ -587,12 +587,18 @@ int gbh_device_open(const char* selector, int allow_software, gbh_device** out) 
         int rt_ver = 0, drv_ver = 0;
         (void)hipRuntimeGetVersion(&rt_ver);
         (void)hipDriverGetVersion(&drv_ver);
+        // adapter name: the marketing name when the runtime reports one, else the ISA
+        std::string arch(p.gcnArchName);
+        std::string name(p.name);
+        if (name.empty()) name = "AMD " + arch.substr(0, arch.find(':'));
+        for (char& ch : name)
+            if (ch == '"' || ch == '\\') ch = ' ';
         char buf[1024];
         snprintf(buf, sizeof buf,
                  "{\"index\": %d, \"adapter_count\": %d, \"name\": \"%s\", \"backend\": \"ROCm-HIP\", "
                  "\"device_type\": \"DiscreteGpu\", \"driver\": \"HIP %d\", \"driver_info\": \"%s, "
                  "%d CUs, driver %d\", \"software\": false}",
-                 index, count, p.name, rt_ver, p.gcnArchName, p.multiProcessorCount, drv_ver);
+                 index, count, name.c_str(), rt_ver, p.gcnArchName, p.multiProcessorCount, drv_ver);
         d->info = buf;
         *out = d;
         return 0;
@@ -1174,6 +1180,23 @@ int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[3], uint32_t first_b
 }
 
 uint64_t gbh_tree_buffer_bytes(gbh_tree* t) { return t ? (uint64_t)t->bytes : 0; }
+
+int gbh_fnv_mix64(uint64_t seed, const uint32_t* words, uint64_t count, uint64_t* out) {
+    if (!out || (count && !words)) {
+        set_error("null argument");
+        return -1;
+    }
+    // checksum_word (galaxy-bh-gpu-tree-parallel.rs:145-149) over each word widened to 64 bits
+    uint64_t h = seed;
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t w = words[i];
+        h ^= w;
+        h *= 0x00000100000001b3ull;
+        h ^= (w << 23) | (w >> 41);
+    }
+    *out = h;
+    return 0;
+}
 
 uint64_t gbh_sparse_layout_bytes(uint32_t n) {
     // nbody_parallel.rs:485-491: bodies + 2 entry buffers + 17N cells + meta

@@ -164,6 +164,10 @@ int gbh_tree_read_flat(gbh_tree* tree, GbhGpuEntry* entries, GbhGpuCell* cells,
 double gbh_tree_last_force_ms(gbh_tree* tree);
 /* buffer_bytes (nbody_parallel.rs:990-992): bytes this library actually holds. */
 uint64_t gbh_tree_buffer_bytes(gbh_tree* tree);
+/* FNV-mix64 of the receipt evidence (checksum_word, galaxy-bh-gpu-tree-parallel.rs:
+ * 145-149) folded over `count` 32-bit words from `seed`; parallel_tree_checksum
+ * (:150-187) is this over the meta, entry and active-cell words in order. */
+int gbh_fnv_mix64(uint64_t seed, const uint32_t* words, uint64_t count, uint64_t* out);
 /* The reference's sparse-layout size for n bodies (nbody_parallel.rs:485-491),
  * kept for receipt-field compatibility. */
 uint64_t gbh_sparse_layout_bytes(uint32_t n);
