@@ -108,6 +108,13 @@ def lib():
         "gbh_fnv_mix64": ([u64, vp, u64, vp], i32),
         "gbh_traverse_packed": ([vp, vp, vp, vp, vp, vp, vp], i32),
         "gbh_force_packed": ([vp, vp, vp, vp, vp, vp], i32),
+        "gbh_host_flat_tree": ([u32, vp, vp, vp, vp, vp, vp, vp, vp, u32, vp], i32),
+        "gbh_gpu_tree_create": ([vp, vp, vp], i32),
+        "gbh_gpu_tree_destroy": ([vp], None),
+        "gbh_gpu_tree_rebuild": ([vp, vp, vp, vp], i32),
+        "gbh_gpu_tree_force": ([vp, vp, vp, vp], i32),
+        "gbh_gpu_tree_read_evidence": ([vp, vp, vp, vp, u32, vp], i32),
+        "gbh_gpu_tree_buffer_bytes": ([vp], u64),
         "gbh_batch_create": ([vp, u32, vp, vp, vp, vp, vp], i32),
         "gbh_batch_destroy": ([vp], None),
         "gbh_batch_step": ([vp, u32], i32),

@@ -140,6 +140,34 @@ class NbodyGpu:
         return {"energy": float(out[0]), "lz": float(out[1]), "kinetic": float(out[2])}
 
     # -- BH #2B1 / #2B2 host-tree ingest (nbody_gpu.rs:640-938) --
+    # -- BH #2C GPU tree (nbody_gpu.rs:1145-1431) --
+    def create_gpu_tree(self, evolving: "EvolvingState") -> "GpuTree":
+        return GpuTree(self, evolving)
+
+    def rebuild_gpu_tree(self, evolving: "EvolvingState", tree: "GpuTree", config: "Config") -> dict:
+        t = L.BuildTiming()
+        _check(L.lib().gbh_gpu_tree_rebuild(evolving._h, tree._h, C.byref(config.c()), C.byref(t)))
+        return t.as_dict()
+
+    def force_from_gpu_tree(self, evolving: "EvolvingState", tree: "GpuTree", config: "Config") -> dict:
+        t = L.StageTiming()
+        _check(L.lib().gbh_gpu_tree_force(evolving._h, tree._h, C.byref(config.c()), C.byref(t)))
+        return t.as_dict()
+
+    def read_gpu_tree_evidence(self, tree: "GpuTree"):
+        """GpuTreeEvidence (nbody_gpu.rs:119-123) -> (meta u32[8], entries u32[n, 4],
+        cells u32[k, 16] in level order)."""
+        meta = np.zeros(8, np.uint32)
+        cap = 16 * tree.count + 1
+        entries = np.zeros((tree.count, 4), np.uint32)
+        cells = np.zeros((cap, 16), np.uint32)
+        k = C.c_uint32()
+        _check(L.lib().gbh_gpu_tree_read_evidence(tree._h, _p(meta), _p(entries), _p(cells), cap, C.byref(k)))
+        return meta, entries, cells[: k.value]
+
+    def gpu_tree_buffer_bytes(self, tree: "GpuTree") -> int:
+        return int(L.lib().gbh_gpu_tree_buffer_bytes(tree._h))
+
     def traverse(self, settings, bodies, entries, cells) -> np.ndarray:
         settings = np.ascontiguousarray(settings, np.uint32)
         bodies = np.ascontiguousarray(bodies, np.uint32)
@@ -157,6 +185,25 @@ class NbodyGpu:
         cells = np.ascontiguousarray(cells, np.uint32)
         _check(L.lib().gbh_force_packed(evolving._h, _p(settings), _p(bodies), _p(entries), _p(cells),
                                         None))
+
+
+class GpuTree:
+    """BH #2C correctness-first GPU tree (GpuTree, nbody_gpu.rs:107-116)."""
+
+    def __init__(self, gpu: NbodyGpu, evolving: "EvolvingState"):
+        h = C.c_void_p()
+        _check(L.lib().gbh_gpu_tree_create(gpu._h, evolving._h, C.byref(h)))
+        self._h = h
+        self.count = evolving.count
+        self._keep = evolving
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gbh_gpu_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class EvolvingState:
@@ -285,6 +332,23 @@ class ParallelTreeRuntime:
 
     def buffer_bytes(self, tree: ParallelGpuTree) -> int:
         return int(L.lib().gbh_tree_buffer_bytes(tree._h))
+
+
+def host_flat_tree(x, y, mass, config: Config):
+    """build_flat_tree + pack_flat_tree on the host (nbody/src/flat.rs:127-268,
+    nbody_gpu.rs:175-250) -> (settings u32[8], bodies u32[n, 8], entries u32[n, 4],
+    cells u32[k, 16]), the records gbh_force_packed takes."""
+    x, y, mass = (np.ascontiguousarray(v, np.float64) for v in (x, y, mass))
+    n = len(x)
+    cap = 16 * n + 1
+    settings = np.zeros(8, np.uint32)
+    bodies = np.zeros((n, 8), np.uint32)
+    entries = np.zeros((n, 4), np.uint32)
+    cells = np.zeros((cap, 16), np.uint32)
+    k = C.c_uint32()
+    _check(L.lib().gbh_host_flat_tree(n, _p(x), _p(y), _p(mass), C.byref(config.c()), _p(settings), _p(bodies),
+                                      _p(entries), _p(cells), cap, C.byref(k)))
+    return settings, bodies, entries, cells[: k.value]
 
 
 def sparse_layout_bytes(n: int) -> int:

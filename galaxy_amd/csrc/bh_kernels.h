@@ -564,6 +564,19 @@ __global__ void k_reset_ctl(uint32_t* __restrict__ ctl, uint32_t first, uint32_t
     if (threadIdx.x < count) ctl[first + threadIdx.x] = 0u;
 }
 
+// Sticky error words of a set into one array: out[0] = the set's control error
+// word, out[1 + s] = simulation s's (so the host reads them with one copy).
+__global__ __launch_bounds__(256) void k_collect_errors(const SimDesc* __restrict__ sims, uint32_t nsims,
+                                                        const uint32_t* __restrict__ ctl,
+                                                        uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) out[0] = ctl[kCtlError];
+    if (i < nsims) {
+        const SimMeta* m = sims[i].meta;
+        out[1 + i] = m ? m->err : 0u;
+    }
+}
+
 // ----------------------------------------------------------------------------
 // K2: root bounds (root_bounds, lib.rs:137-154), Morton keys (flat.rs:224-231),
 // per-pass digit histograms, look-back flag reset, per-depth thresholds.

@@ -153,8 +153,9 @@ struct SimSet {
     unsigned long long* d_tflags = nullptr;
     uint32_t* d_tsum = nullptr;  // per topology tile boundary summaries (k_topo_pre)
     uint32_t* d_psum = nullptr;  // and their scans over the simulation's tiles (k_topo_scan)
+    uint32_t* d_errs = nullptr;  // collected error words (k_collect_errors)
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
-    size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0;
+    size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0, cap_errs = 0;
     uint32_t sort_kpt = 8;
     uint32_t trav_block = 256;        // traversal workgroup size (64, 128 or 256)
     uint32_t trav_waves_per_cu = 32;  // persistent traversal wavefronts per CU
@@ -165,14 +166,14 @@ struct SimSet {
     void release() {
         for (void* p : {(void*)d_desc, (void*)d_t256, (void*)d_tsort, (void*)d_ttopo,
                         (void*)d_twave, (void*)d_ctl, (void*)d_sflags, (void*)d_tflags,
-                        (void*)d_tsum, (void*)d_psum})
+                        (void*)d_tsum, (void*)d_psum, (void*)d_errs})
             if (p) (void)hipFree(p);
         d_desc = nullptr;
         d_t256 = d_tsort = d_ttopo = d_twave = nullptr;
-        d_ctl = d_sflags = d_tsum = d_psum = nullptr;
+        d_ctl = d_sflags = d_tsum = d_psum = d_errs = nullptr;
         d_tflags = nullptr;
         cap_desc = cap_t256 = cap_tsort = cap_ttopo = cap_twave = cap_sflags = cap_tflags = 0;
-        cap_tsum = cap_psum = 0;
+        cap_tsum = cap_psum = cap_errs = 0;
     }
 
     template <class T>
@@ -328,17 +329,22 @@ struct SimSet {
     }
 
     // Sticky error words of the set and of each simulation.
+    // One gather kernel and one copy, whatever the number of simulations.
     uint32_t read_errors(hipStream_t s, std::vector<uint32_t>* per_sim = nullptr) {
-        uint32_t ctl_err = 0;
-        HIP_OK(hipMemcpyAsync(&ctl_err, d_ctl + kCtlError, 4, hipMemcpyDeviceToHost, s));
-        std::vector<uint32_t> errs(desc.size(), 0);
-        for (size_t i = 0; i < desc.size(); ++i)
-            if (desc[i].meta)
-                HIP_OK(hipMemcpyAsync(&errs[i], &desc[i].meta->err, 4, hipMemcpyDeviceToHost, s));
+        const uint32_t nsims = (uint32_t)desc.size();
+        ensure(d_errs, cap_errs, 1ull + nsims);
+        hipLaunchKernelGGL(k_collect_errors, dim3(div_up(nsims + 1u, 256u)), dim3(256), 0, s, d_desc, nsims,
+                           d_ctl, d_errs);
+        HIP_OK(hipGetLastError());
+        std::vector<uint32_t> words(1ull + nsims, 0);
+        HIP_OK(hipMemcpyAsync(words.data(), d_errs, 4ull * words.size(), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        uint32_t all = ctl_err;
-        for (uint32_t e : errs) all |= e;
-        if (per_sim) *per_sim = errs;
+        uint32_t all = 0;
+        for (uint32_t e : words) all |= e;
+        if (per_sim) {  // a set-wide error (words[0]) taints every simulation of the set
+            per_sim->resize(nsims);
+            for (uint32_t i = 0; i < nsims; ++i) (*per_sim)[i] = words[1 + i] | words[0];
+        }
         return all;
     }
 };
@@ -1209,4 +1215,6 @@ uint64_t gbh_sparse_layout_bytes(uint32_t n) {
 }  // extern "C"
 
 #include "bh_packed.inc"
+#include "bh_gpu_tree.inc"
+#include "bh_host_tree.inc"
 #include "bh_serve.inc"

@@ -8,11 +8,17 @@ FNV-mix64 tree checksum of the initial, final and same-state repeat trees, the
 direct-probe force gate, the complete-rebuild benchmark (upper median) and a
 receipt with schema galaxy.barnes-hut-parallel-gpu-tree-receipt.v1.
 
+Up to --oracle-limit bodies it also runs the reference's two GPU oracles
+(galaxy-bh-gpu-tree-parallel.rs:422-474): the same trajectory through the BH #2C
+correctness-first GPU tree (run_serial_oracle, :225-246) and through host-built
+BH #2A trees fed to the traversal (BH #2B2, run_host_oracle, :248-271), with the
+reference's state and force gates; and up to 4,096 bodies the BH #2C rebuild
+benchmark beside the BH #2D one (:484-505).
+
 Differences, each recorded in the receipt itself:
-  * the CPU BH #2A flat tree, its force and trajectory comparisons and the BH #2C /
-    BH #2B2 GPU oracles are not part of this library: the CPU restatement is the
-    test oracle, and tests/ compare this path with it bit for bit.  Those sections
-    carry status "skipped-test-oracle";
+  * the CPU BH #2A f64 force and trajectory comparisons (:387-420) carry status
+    "skipped-test-oracle": the CPU f64 restatement is the test oracle, and tests/
+    compare this path with it bit for bit and attach those sections;
   * parallel_tree_buffer_bytes keeps the reference's sparse-layout formula
     (17 N cells of 64 bytes, nbody_parallel.rs:990-992) for compatibility;
     device_tree_buffer_bytes is what this build allocates;
@@ -34,7 +40,7 @@ import numpy as np
 
 from . import _lib as L
 from . import ics
-from .bh import BhError, Config, NbodyGpu, ParallelTreeRuntime
+from .bh import BhError, Config, NbodyGpu, ParallelTreeRuntime, host_flat_tree
 
 RECEIPT_SCHEMA = "galaxy.barnes-hut-parallel-gpu-tree-receipt.v1"
 TREE_ORDERING = "stable-lsd-radix-4bit-morton-code; resident body index retained for equal Morton keys"
@@ -74,6 +80,69 @@ def validate_parallel_order(ev):
     if bad.size:
         a, b = ev.entries[bad[0], :2], ev.entries[bad[0] + 1, :2]
         raise BhError(f"BH #2D radix order is not strict by (Morton code, body index): {list(a)} then {list(b)}")
+
+
+def state_error(approx: np.ndarray, ref: np.ndarray) -> dict:
+    """state_error (galaxy-bh-gpu-tree-parallel.rs:62-90) of two (n, 8) states, as
+    the receipt's error4 object (:92-99)."""
+    a = np.asarray(approx, np.float64)
+    b = np.asarray(ref, np.float64)
+    pd = np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1])
+    vd = np.hypot(a[:, 4] - b[:, 4], a[:, 5] - b[:, 5])
+    pr = np.hypot(b[:, 0], b[:, 1])
+    vr = np.hypot(b[:, 4], b[:, 5])
+    return {"position_rms_relative_l2": float(np.sqrt(np.sum(pd * pd) / max(np.sum(pr * pr), 1e-30))),
+            "position_max_relative": float(np.max(pd / np.maximum(pr, 1e-6))),
+            "velocity_rms_relative_l2": float(np.sqrt(np.sum(vd * vd) / max(np.sum(vr * vr), 1e-30))),
+            "velocity_max_relative": float(np.max(vd / np.maximum(vr, 1e-9)))}
+
+
+def force_error(approx: np.ndarray, ref: np.ndarray):
+    """force_error (galaxy-bh-gpu-tree-parallel.rs:101-114) of two (n, 4) accelerations."""
+    a = np.asarray(approx, np.float64)
+    b = np.asarray(ref, np.float64)
+    rel = np.hypot(a[:, 0] - b[:, 0], a[:, 1] - b[:, 1]) / np.maximum(np.hypot(b[:, 0], b[:, 1]), 1e-30)
+    return float(np.sqrt(np.sum(rel * rel) / len(rel))), float(np.max(rel))
+
+
+def state_gate(name: str, e: dict):
+    if (e["position_rms_relative_l2"] >= 0.03 or e["position_max_relative"] >= 0.30
+            or e["velocity_rms_relative_l2"] >= 0.03 or e["velocity_max_relative"] >= 0.30):
+        raise BhError(f"{name} parity failed: {e}")
+
+
+def run_serial_oracle(gpu: NbodyGpu, initial: np.ndarray, steps: int, dt: float, config: Config):
+    """run_serial_oracle (galaxy-bh-gpu-tree-parallel.rs:225-246): the trajectory on
+    the BH #2C correctness-first GPU tree."""
+    ev = gpu.create_evolving_state(initial)
+    tree = gpu.create_gpu_tree(ev)
+    gpu.rebuild_gpu_tree(ev, tree, config)
+    gpu.force_from_gpu_tree(ev, tree, config)
+    for _ in range(steps):
+        gpu.kick_drift(ev, dt)
+        gpu.rebuild_gpu_tree(ev, tree, config)
+        gpu.force_from_gpu_tree(ev, tree, config)
+        gpu.final_kick(ev, dt)
+    out = gpu.read_evolving_state(ev), gpu.read_evolving_accelerations(ev)
+    tree.close()
+    ev.close()
+    return out
+
+
+def run_host_oracle(gpu: NbodyGpu, bodies: np.ndarray, steps: int, dt: float, config: Config):
+    """run_host_oracle (galaxy-bh-gpu-tree-parallel.rs:248-271): BH #2A trees built on
+    the host (the first from the f64 bodies, then from each read-back drifted
+    state) and traversed on the GPU (BH #2B2)."""
+    ev = gpu.create_evolving_state(ics.bodies_to_state(bodies))
+    gpu.force_into(ev, *host_flat_tree(bodies[:, 0], bodies[:, 1], bodies[:, 4], config))
+    for _ in range(steps):
+        gpu.kick_drift(ev, dt)
+        d = gpu.read_evolving_state(ev).astype(np.float64)
+        gpu.force_into(ev, *host_flat_tree(d[:, 0], d[:, 1], d[:, 2], config))
+        gpu.final_kick(ev, dt)
+    out = gpu.read_evolving_state(ev), gpu.read_evolving_accelerations(ev)
+    ev.close()
+    return out
 
 
 def upper_median(values):
@@ -137,7 +206,8 @@ def run(a, keep: dict | None = None) -> dict:
     """Runs the workload and returns the receipt; `keep` (optional) receives the
     initial state and the final state and accelerations."""
     check_args(a)
-    initial = ics.bodies_to_state(make_bodies(a.preset, a.particles, a.seed))
+    bodies = make_bodies(a.preset, a.particles, a.seed)
+    initial = ics.bodies_to_state(bodies)
     config = Config(theta=a.theta, softening_kpc=a.softening_kpc, g=G_KPC3_PER_MSUN_MYR2, bucket=4,
                     max_depth=48)
     gpu = NbodyGpu(a.adapter, a.allow_software)
@@ -209,6 +279,48 @@ def run(a, keep: dict | None = None) -> dict:
         samples.append(time.perf_counter() - t0)
     parallel_median = upper_median(samples)
 
+    # the GPU oracles (galaxy-bh-gpu-tree-parallel.rs:422-474)
+    t0 = time.perf_counter()
+    if a.particles <= a.oracle_limit:
+        serial_state, serial_accel = run_serial_oracle(gpu, initial, a.steps, a.dt_myr, config)
+        host_state, host_accel = run_host_oracle(gpu, bodies, a.steps, a.dt_myr, config)
+        serial_err = state_error(final_state, serial_state)
+        host_err = state_error(final_state, host_state)
+        state_gate("BH #2C state", serial_err)
+        state_gate("BH #2B2 state", host_err)
+        s_rms, s_max = force_error(final_accel, serial_accel)
+        h_rms, h_max = force_error(final_accel, host_accel)
+        if s_rms >= 0.04 or s_max >= 0.30 or h_rms >= 0.04 or h_max >= 0.30:
+            raise BhError("BH #2D force parity failed against BH #2C/BH #2B2")
+        gpu_oracles = {
+            "status": "executed",
+            "bh2c_serial_gpu": {"state_error": serial_err, "force_rms_relative": s_rms,
+                                "force_max_relative": s_max},
+            "bh2b2_host_tree_gpu": {"state_error": host_err, "force_rms_relative": h_rms,
+                                    "force_max_relative": h_max},
+        }
+    else:
+        gpu_oracles = {"status": "skipped-particle-limit", "limit": a.oracle_limit,
+                       "reason": "BH #2C serialized GPU oracle is intentionally capped at 4096 bodies"}
+    gpu_oracle_s = time.perf_counter() - t0
+
+    # the BH #2C rebuild beside the BH #2D one (galaxy-bh-gpu-tree-parallel.rs:484-505)
+    if a.particles <= 4096:
+        serial_tree = gpu.create_gpu_tree(evolving)
+        for _ in range(a.benchmark_warmup):
+            gpu.rebuild_gpu_tree(evolving, serial_tree, config)
+        serial_samples = []
+        for _ in range(a.benchmark_repeats):
+            t0 = time.perf_counter()
+            gpu.rebuild_gpu_tree(evolving, serial_tree, config)
+            serial_samples.append(time.perf_counter() - t0)
+        serial_tree.close()
+        serial_median = upper_median(serial_samples)
+        bh2c_serial = {"status": "executed", "samples_seconds": serial_samples, "median_seconds": serial_median,
+                       "parallel_vs_serial_speedup": serial_median / parallel_median}
+    else:
+        bh2c_serial = {"status": "skipped-bh2c-cap", "limit": 4096}
+
     oracle_skip = dict(ORACLE_SKIP, limit=a.oracle_limit)
     build_total = sum(build.values())
     receipt = {
@@ -254,7 +366,7 @@ def run(a, keep: dict | None = None) -> dict:
             "direct_probe_max_relative": direct["max_relative"],
         },
         "trajectory_vs_bh2a_flat_f64": oracle_skip,
-        "gpu_oracles": oracle_skip,
+        "gpu_oracles": gpu_oracles,
         "tree_build_benchmark": {
             "sample_scope": BENCHMARK_SAMPLE_SCOPE,
             "stage_timings_are_diagnostics": True,
@@ -262,8 +374,7 @@ def run(a, keep: dict | None = None) -> dict:
             "repeats": a.benchmark_repeats,
             "parallel_samples_seconds": samples,
             "parallel_median_seconds": parallel_median,
-            "bh2c_serial": {"status": "skipped-not-built",
-                            "reason": "the BH #2C serialized builder is not part of this library"},
+            "bh2c_serial": bh2c_serial,
         },
         "timings_seconds": {
             "gpu_state_create_transfer": state_create_s,
@@ -283,6 +394,7 @@ def run(a, keep: dict | None = None) -> dict:
             "final_state_readback": state_readback_s,
             "final_acceleration_readback": accel_readback_s,
             "direct_probe_reference": direct_s,
+            "gpu_oracles_total": gpu_oracle_s,
             "total_wall": time.perf_counter() - wall0,
         },
         "scope": "BH #2D resident step on gfx950: f64 root bounds and Morton keys, onesweep 8-bit radix "

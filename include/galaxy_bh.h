@@ -184,6 +184,46 @@ int gbh_force_packed(gbh_state* st, const GbhBhSettings* settings, const GbhGpuB
                      const GbhGpuEntry* entries, const GbhGpuCell* cells,
                      gbh_stage_timing* timing);
 
+/* Host BH #2A tree packed for the GPU: build_flat_tree (nbody/src/flat.rs:127-268)
+ * of the f64 bodies (x, y, mass), then pack_flat_tree (nbody_gpu.rs:175-250) into
+ * settings, bodies[n], entries[n] and cells (preorder, root 0).  *cell_count
+ * receives the tree's cell count; the call fails if cap is smaller (at most
+ * 16 n + 1 cells).  The packed records feed gbh_force_packed (BH #2B2). */
+int gbh_host_flat_tree(uint32_t n, const double* x, const double* y, const double* mass,
+                       const gbh_config* cfg, GbhBhSettings* settings, GbhGpuBody* bodies,
+                       GbhGpuEntry* entries, GbhGpuCell* cells, uint32_t cap,
+                       uint32_t* cell_count);
+
+/* ---------------------------------------------------------------------------
+ * BH #2C correctness-first GPU tree (NbodyGpu::create_gpu_tree, rebuild_gpu_tree,
+ * force_from_gpu_tree, read_gpu_tree_evidence, gpu_tree_buffer_bytes;
+ * nbody_gpu.rs:1145-1431; kernels nbody_tree.wgsl:77-370): f32 root bounds,
+ * keys and (code, index) order, cells in level order, f32 aggregates; the force
+ * follows bh_traverse (nbody.wgsl:43-123) over these cells.  Used as a GPU
+ * oracle of the BH #2D path (galaxy-bh-gpu-tree-parallel.rs:225-246, 422-474).
+ * ------------------------------------------------------------------------- */
+typedef struct gbh_gpu_tree gbh_gpu_tree;
+/* create_gpu_tree (nbody_gpu.rs:1146): MAX_GPU_TREE_BODIES */
+#define GBH_MAX_GPU_TREE_BODIES 4096u
+/* create_gpu_tree (nbody_gpu.rs:1145-1212): 2..=4096 resident bodies, cell
+ * capacity 16 n + 1. */
+int gbh_gpu_tree_create(gbh_device* dev, gbh_state* st, gbh_gpu_tree** out);
+void gbh_gpu_tree_destroy(gbh_gpu_tree* tree);
+/* rebuild_gpu_tree (nbody_gpu.rs:1214-1301).  The six stages run as one
+ * launch; their wall time is reported in timing->bounds_seconds (the slot
+ * order of TreeBuildTiming, nbody_gpu.rs:85-93, with sort in radix_seconds). */
+int gbh_gpu_tree_rebuild(gbh_state* st, gbh_gpu_tree* tree, const gbh_config* cfg,
+                         gbh_build_timing* timing);
+/* force_from_gpu_tree (nbody_gpu.rs:1303-1366): writes the state's acceleration. */
+int gbh_gpu_tree_force(gbh_state* st, gbh_gpu_tree* tree, const gbh_config* cfg,
+                       gbh_stage_timing* timing);
+/* read_gpu_tree_evidence (nbody_gpu.rs:1392-1428): meta, the n sorted entries and
+ * the meta.data[0] cells in level order (cells must hold cap >= that many). */
+int gbh_gpu_tree_read_evidence(gbh_gpu_tree* tree, GbhTreeMeta* meta, GbhGpuEntry* entries,
+                               GbhGpuCell* cells, uint32_t cap, uint32_t* cell_count);
+/* gpu_tree_buffer_bytes (nbody_gpu.rs:1430-1432): bytes this library holds. */
+uint64_t gbh_gpu_tree_buffer_bytes(gbh_gpu_tree* tree);
+
 /* ---------------------------------------------------------------------------
  * Batched ensembles (no reference counterpart): many independent simulations of
  * uneven N advanced together, one launch per phase, whole step captured in a

@@ -1014,3 +1014,180 @@ void cref_state_error(uint32_t n, const double* ax, const double* ay, const doub
     out[2] = sqrt(v_err / rmax(v_ref, 1e-30));
     out[3] = v_max;
 }
+
+/* ============ BH #2C correctness-first GPU tree (nbody_tree.wgsl) =========
+ * The reference's serialized algorithm, step for step, in f32:
+ * bh_tree_bounds (:77-108), bh_tree_morton (:110-131), bh_tree_sort (:133-167,
+ * bitonic network over the padded entries with the (code, index) order of
+ * :72-78), bh_tree_positions (:169-176), bh_tree_topology (:204-302, level by
+ * level, children in quadrant order), bh_tree_aggregate (:304-370, reverse
+ * cell order).
+ */
+static float quantize_f32(float value, float center, float half) {
+    float low = center - half;
+    float width = 2.0f * half;
+    float scaled = floorf((value - low) / width * 65536.0f);
+    if (!(scaled > 0.0f)) scaled = 0.0f; /* clamp(scaled, 0, 65535) */
+    if (scaled > 65535.0f) scaled = 65535.0f;
+    return scaled;
+}
+
+static int entry_after(const cref_gpu_entry* a, const cref_gpu_entry* b) {
+    return a->data[0] > b->data[0] || (a->data[0] == b->data[0] && a->data[1] > b->data[1]);
+}
+
+static int entry_before(const cref_gpu_entry* a, const cref_gpu_entry* b) {
+    return a->data[0] < b->data[0] || (a->data[0] == b->data[0] && a->data[1] < b->data[1]);
+}
+
+static cref_gpu_cell empty_cell_f32(float cx, float cy, float half, uint32_t start, uint32_t end,
+                                    uint32_t depth) {
+    cref_gpu_cell c;
+    memset(&c, 0, sizeof c);
+    c.center_half_mass[0] = cx;
+    c.center_half_mass[1] = cy;
+    c.center_half_mass[2] = half;
+    c.range_depth[0] = start;
+    c.range_depth[1] = end;
+    c.range_depth[2] = depth;
+    for (int q = 0; q < 4; ++q) c.children[q] = CREF_NO_CHILD;
+    return c;
+}
+
+int cref_gpu_tree_build(uint32_t n, const cref_evolve_state* states, uint32_t bucket,
+                        uint32_t max_depth, cref_tree_meta* meta, cref_gpu_body* bodies,
+                        cref_gpu_entry* entries, uint32_t padded, cref_gpu_cell* cells,
+                        uint32_t capacity) {
+    if (n < 2 || padded < n || (padded & (padded - 1)) != 0 || capacity < 1) {
+        set_error("invalid BH #2C tree sizes");
+        return -1;
+    }
+    /* bounds */
+    float min_x = states[0].position_mass[0], max_x = min_x;
+    float min_y = states[0].position_mass[1], max_y = min_y;
+    for (uint32_t i = 1; i < n; ++i) {
+        float x = states[i].position_mass[0], y = states[i].position_mass[1];
+        min_x = fminf(min_x, x);
+        max_x = fmaxf(max_x, x);
+        min_y = fminf(min_y, y);
+        max_y = fmaxf(max_y, y);
+    }
+    float span = fmaxf(fmaxf(max_x - min_x, max_y - min_y), 1.0e-12f);
+    memset(meta, 0, sizeof *meta);
+    meta->bounds[0] = 0.5f * (min_x + max_x);
+    meta->bounds[1] = 0.5f * (min_y + max_y);
+    meta->bounds[2] = 0.5f * span + 1.0e-12f;
+    meta->bounds[3] = 0.0f;
+    /* Morton entries and body copies */
+    for (uint32_t i = 0; i < padded; ++i) {
+        memset(&entries[i], 0, sizeof entries[i]);
+        if (i >= n) {
+            entries[i].data[0] = 0xffffffffu;
+            entries[i].data[1] = 0xffffffffu;
+            continue;
+        }
+        const float* pm = states[i].position_mass;
+        uint32_t qx = (uint32_t)quantize_f32(pm[0], meta->bounds[0], meta->bounds[2]);
+        uint32_t qy = (uint32_t)quantize_f32(pm[1], meta->bounds[1], meta->bounds[2]);
+        entries[i].data[0] = morton2(qx, qy);
+        entries[i].data[1] = i;
+        memcpy(bodies[i].position_mass, pm, sizeof bodies[i].position_mass);
+        memset(bodies[i].meta, 0, sizeof bodies[i].meta);
+    }
+    /* one-invocation bitonic sort */
+    for (uint32_t k = 2; k <= padded; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = 0; i < padded; ++i) {
+                uint32_t ixj = i ^ j;
+                if (ixj > i && ixj < padded) {
+                    int ascending = (i & k) == 0;
+                    cref_gpu_entry a = entries[i], b = entries[ixj];
+                    int swap = ascending ? entry_after(&a, &b) : entry_before(&a, &b);
+                    if (swap) {
+                        entries[i] = b;
+                        entries[ixj] = a;
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t p = 0; p < n; ++p) bodies[entries[p].data[1]].meta[0] = p;
+    /* topology */
+    uint32_t md = max_depth < 16u ? max_depth : 16u;
+    cells[0] = empty_cell_f32(meta->bounds[0], meta->bounds[1], meta->bounds[2], 0, n, 0);
+    uint32_t next_free = 1, level_start = 0, level_end = 1, leaf_count = 0, max_seen = 0;
+    while (level_start < level_end) {
+        for (uint32_t pi = level_start; pi < level_end; ++pi) {
+            cref_gpu_cell parent = cells[pi];
+            uint32_t start = parent.range_depth[0], end = parent.range_depth[1];
+            uint32_t depth = parent.range_depth[2];
+            if (depth > max_seen) max_seen = depth;
+            if (end - start <= bucket || depth >= md) {
+                leaf_count += 1;
+                continue;
+            }
+            uint32_t shift = 30u - 2u * depth;
+            uint32_t cursor = start;
+            while (cursor < end) {
+                uint32_t quadrant = (entries[cursor].data[0] >> shift) & 3u;
+                uint32_t group_start = cursor;
+                cursor += 1;
+                while (cursor < end && ((entries[cursor].data[0] >> shift) & 3u) == quadrant) cursor += 1;
+                if (next_free >= capacity) {
+                    meta->data[0] = next_free;
+                    meta->data[1] = leaf_count;
+                    meta->data[2] = max_seen;
+                    meta->data[3] = 1;
+                    return 0;
+                }
+                float h = 0.5f * parent.center_half_mass[2];
+                float ox = (quadrant & 1u) ? h : -h;
+                float oy = (quadrant & 2u) ? h : -h;
+                cells[next_free] = empty_cell_f32(parent.center_half_mass[0] + ox,
+                                                  parent.center_half_mass[1] + oy, h, group_start,
+                                                  cursor, depth + 1u);
+                parent.children[quadrant] = next_free;
+                next_free += 1;
+            }
+            cells[pi] = parent;
+        }
+        if (next_free == level_end) break;
+        level_start = level_end;
+        level_end = next_free;
+    }
+    meta->data[0] = next_free;
+    meta->data[1] = leaf_count;
+    meta->data[2] = max_seen;
+    meta->data[3] = 0;
+    /* aggregates, reverse cell order */
+    for (uint32_t r = next_free; r > 0; --r) {
+        cref_gpu_cell* cell = &cells[r - 1];
+        int leaf = cell->children[0] == CREF_NO_CHILD && cell->children[1] == CREF_NO_CHILD &&
+                   cell->children[2] == CREF_NO_CHILD && cell->children[3] == CREF_NO_CHILD;
+        float mass = 0.0f, wx = 0.0f, wy = 0.0f;
+        if (leaf) {
+            for (uint32_t p = cell->range_depth[0]; p < cell->range_depth[1]; ++p) {
+                const float* b = bodies[entries[p].data[1]].position_mass;
+                mass = mass + b[2];
+                wx = wx + b[2] * b[0];
+                wy = wy + b[2] * b[1];
+            }
+        } else {
+            for (int q = 0; q < 4; ++q) {
+                uint32_t ci = cell->children[q];
+                if (ci != CREF_NO_CHILD) {
+                    float cm = cells[ci].center_half_mass[3];
+                    mass = mass + cm;
+                    wx = wx + cm * cells[ci].com[0];
+                    wy = wy + cm * cells[ci].com[1];
+                }
+            }
+        }
+        cell->center_half_mass[3] = mass;
+        cell->com[0] = wx / mass;
+        cell->com[1] = wy / mass;
+        cell->com[2] = 0.0f;
+        cell->com[3] = 0.0f;
+    }
+    return 0;
+}

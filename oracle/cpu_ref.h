@@ -13,6 +13,7 @@
  *   - BH #2B1 packing into the f32/u32 transfer ABI ...... runtime/src/nbody_gpu.rs:175-250
  *   - BH #2B1/2D f32 flat traversal ...................... runtime/src/nbody.wgsl:35-123
  *   - BH #2B2 f32 kick/drift and final kick .............. runtime/src/nbody_evolve.wgsl:19-46
+ *   - BH #2C correctness-first GPU tree (f32) ........... runtime/src/nbody_tree.wgsl:77-370
  *   - BH #2D sparse level-order evidence + checksum ...... runtime/src/nbody_parallel.rs:929-988,
  *                                                         runtime/src/bin/galaxy-bh-gpu-tree-parallel.rs:150-203
  *   - error metrics / cpu_flat_evolve .................... galaxy-bh-gpu-tree-parallel.rs:62-148
@@ -131,6 +132,14 @@ void cref_energy(uint32_t n, const double* x, const double* y, const double* vx,
 
 /* ---- integration -------------------------------------------------------- */
 /* bh_kick_drift / bh_final_kick (nbody_evolve.wgsl:19-46) on the frozen records */
+/* BH #2C correctness-first GPU tree (runtime/src/nbody_tree.wgsl:77-370) on f32
+ * states: meta.data = {cells, leaves, max depth seen, overflow}, meta.bounds =
+ * {cx, cy, half, 0}; bodies[n] with meta[0] = sorted position; entries[padded]
+ * (padded = next power of two >= n); cells in level order (capacity 16n + 1). */
+int cref_gpu_tree_build(uint32_t n, const cref_evolve_state* states, uint32_t bucket,
+                        uint32_t max_depth, cref_tree_meta* meta, cref_gpu_body* bodies,
+                        cref_gpu_entry* entries, uint32_t padded, cref_gpu_cell* cells,
+                        uint32_t capacity);
 void cref_kick_drift_f32(uint32_t n, cref_evolve_state* s, const cref_gpu_accel* a, float dt);
 void cref_final_kick_f32(uint32_t n, cref_evolve_state* s, const cref_gpu_accel* a, float dt);
 /* cpu_flat_evolve (galaxy-bh-gpu-tree-parallel.rs:124-148): f64 BH #2A trajectory.

@@ -86,6 +86,7 @@ def lib():
             "cref_state_checksum": ([u32, vp, vp, vp, vp, vp], u64),
             "cref_force_error": ([u32, vp, vp, vp, vp, vp], None),
             "cref_state_error": ([u32, vp, vp, vp, vp, vp, vp, vp, vp, vp], None),
+            "cref_gpu_tree_build": ([u32, vp, u32, u32, vp, vp, vp, u32, vp, u32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(_lib, name)
@@ -312,3 +313,50 @@ def tree_from_state(state, cfg):
     s = np.asarray(state, dtype=np.float32)
     return FlatTree(s[:, 0].astype(np.float64), s[:, 1].astype(np.float64),
                     s[:, 2].astype(np.float64), cfg)
+
+
+class GpuTreeBuild:
+    """BH #2C tree (runtime/src/nbody_tree.wgsl:77-370) of an f32 state: meta
+    (data u32[4], bounds f32[4]), bodies u32[n, 8], entries u32[n, 4] and the
+    level-order cells u32[cells, 16], in the ABI's words."""
+
+    def __init__(self, state, bucket=4, max_depth=48):
+        s = np.ascontiguousarray(state, np.float32)
+        n = len(s)
+        padded = 1 << max(0, (n - 1).bit_length())
+        cap = 16 * n + 1
+        self.meta = np.zeros(8, np.uint32)
+        self.bodies = np.zeros((n, 8), np.uint32)
+        entries = np.zeros((padded, 4), np.uint32)
+        cells = np.zeros((cap, 16), np.uint32)
+        _check(lib().cref_gpu_tree_build(n, _p(s), bucket, max_depth, _p(self.meta), _p(self.bodies),
+                                         _p(entries), padded, _p(cells), cap))
+        self.n = n
+        self.entries = entries[:n]
+        self.cells = cells[:int(self.meta[0])] if self.meta[3] == 0 else cells[:0]
+
+    def bounds(self):
+        return self.meta[4:8].view(np.float32)
+
+    def settings(self, theta, softening, g):
+        """BhSettings of force_from_gpu_tree (nbody_gpu.rs:1303-1325): root cell 0."""
+        st = np.zeros(8, np.uint32)
+        st[0], st[1], st[2] = self.n, len(self.cells), 0
+        st[4:7] = np.array([theta, softening, g], np.float32).view(np.uint32)
+        return st
+
+    def accelerations(self, theta, softening, g):
+        return traverse_f32(self.settings(theta, softening, g), self.bodies, self.entries, self.cells)
+
+
+def gpu_tree_evolve(state, steps, dt, theta, softening, g, bucket=4, max_depth=48):
+    """run_serial_oracle (galaxy-bh-gpu-tree-parallel.rs:225-246): BH #2C build and
+    force before the loop, then kick/drift, rebuild, force, final kick per step.
+    Returns (final state, last accelerations)."""
+    s = np.array(state, dtype=np.float32, order="C")
+    a = GpuTreeBuild(s, bucket, max_depth).accelerations(theta, softening, g)
+    for _ in range(steps):
+        kick_drift_f32(s, a, dt)
+        a = GpuTreeBuild(s, bucket, max_depth).accelerations(theta, softening, g)
+        final_kick_f32(s, a, dt)
+    return s, a

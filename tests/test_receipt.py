@@ -14,6 +14,7 @@ WORKLOAD = dict(particles=512, preset="disc", steps=3, dt_myr=0.01, seed=303, th
 
 def receipt_for(n=512, steps=3):
     samples = [0.0011, 0.0009, 0.0012, 0.0010, 0.0013]
+    serial = [0.004, 0.0041, 0.0039, 0.0042, 0.0040]
     return {
         "schema": R.RECEIPT_SCHEMA, "status": "complete", "phase": "BH-2D",
         "builder": "gpu-parallel-sparse-radix-v1", "measurement_class": "hardware",
@@ -32,12 +33,23 @@ def receipt_for(n=512, steps=3):
         "final_force": {"bh2a_flat_status": R.TEST_ORACLE_SKIP, "direct_probe_count": 12,
                         "direct_probe_rms_relative": 0.01, "direct_probe_max_relative": 0.02},
         "trajectory_vs_bh2a_flat_f64": {"status": R.TEST_ORACLE_SKIP, "limit": 4096},
-        "gpu_oracles": {"status": R.TEST_ORACLE_SKIP, "limit": 4096},
+        "gpu_oracles": {"status": "executed",
+                        "bh2c_serial_gpu": {"state_error": small_state_error(), "force_rms_relative": 0.011,
+                                            "force_max_relative": 0.05},
+                        "bh2b2_host_tree_gpu": {"state_error": small_state_error(), "force_rms_relative": 0.002,
+                                                "force_max_relative": 0.01}},
         "tree_build_benchmark": {"sample_scope": R.BENCHMARK_SAMPLE_SCOPE, "stage_timings_are_diagnostics": True,
                                  "warmup": 1, "repeats": 5, "parallel_samples_seconds": samples,
                                  "parallel_median_seconds": sorted(samples)[2],
-                                 "bh2c_serial": {"status": "skipped-not-built"}},
+                                 "bh2c_serial": {"status": "executed", "samples_seconds": serial,
+                                                 "median_seconds": sorted(serial)[2],
+                                                 "parallel_vs_serial_speedup": sorted(serial)[2] / sorted(samples)[2]}},
     }
+
+
+def small_state_error():
+    return {"position_rms_relative_l2": 1e-4, "position_max_relative": 1e-3,
+            "velocity_rms_relative_l2": 2e-4, "velocity_max_relative": 3e-3}
 
 
 def test_valid_receipt_passes():
@@ -94,7 +106,13 @@ def mutate(path, value):
     (("final_force", "direct_probe_max_relative"), 0.005, "RMS cannot exceed"),
     (("final_force", "direct_probe_max_relative"), 0.5, "maximum is too large|max gate"),
     (("final_force", "bh2a_flat_status"), "skipped-particle-limit", "final-force status"),
-    (("gpu_oracles", "limit"), 100, "skip limit"),
+    (("gpu_oracles", "status"), R.TEST_ORACLE_SKIP, "GPU oracle status"),
+    (("gpu_oracles", "bh2c_serial_gpu", "force_max_relative"), 0.31, "force gate|too large"),
+    (("gpu_oracles", "bh2b2_host_tree_gpu", "state_error", "position_rms_relative_l2"), 0.05, "RMS gate"),
+    (("gpu_oracles", "bh2c_serial_gpu", "force_rms_relative"), 0.06, "cannot exceed|force gate"),
+    (("tree_build_benchmark", "bh2c_serial", "status"), "skipped-not-built", "should execute"),
+    (("tree_build_benchmark", "bh2c_serial", "median_seconds"), 0.0041, "BH #2C benchmark median"),
+    (("tree_build_benchmark", "bh2c_serial", "parallel_vs_serial_speedup"), 3.0, "speedup"),
     (("tree_build_benchmark", "parallel_median_seconds"), 0.0012, "median"),
     (("tree_build_benchmark", "parallel_samples_seconds"), [0.001], "sample count"),
     (("tree_build_benchmark", "stage_timings_are_diagnostics"), False, "diagnostics"),
@@ -145,3 +163,42 @@ def test_library_checksum_is_fnv_mix64(oracle):
     ev = Ev()
     ev.meta, ev.entries, ev.cells = meta, entries, cells
     assert driver.parallel_tree_checksum(ev) == oracle.parallel_tree_checksum(meta, entries, cells)
+
+def test_above_the_oracle_and_bh2c_caps():
+    n = 5000
+    r = receipt_for(n=n)
+    r["tree"].update(active_cell_count=1800, leaf_count=1300, max_depth=9)
+    r["parallel_tree_buffer_bytes"] = R.expected_parallel_tree_buffer_bytes(n)
+    skip = {"status": "skipped-particle-limit", "limit": 4096}
+    r["final_force"]["bh2a_flat_status"] = "skipped-particle-limit"
+    r["trajectory_vs_bh2a_flat_f64"] = dict(skip)
+    r["gpu_oracles"] = dict(skip)
+    r["tree_build_benchmark"]["bh2c_serial"] = {"status": "skipped-bh2c-cap", "limit": 4096}
+    w = dict(WORKLOAD, particles=n)
+    assert R.validate_receipt(copy.deepcopy(r), **w)["bh2c_parallel_vs_serial_speedup"] is None
+    r["tree_build_benchmark"]["bh2c_serial"]["limit"] = 1000
+    with pytest.raises(R.ReceiptError, match="skip limit"):
+        R.validate_receipt(copy.deepcopy(r), **w)
+    r["tree_build_benchmark"]["bh2c_serial"] = {"status": "executed"}
+    with pytest.raises(R.ReceiptError, match="skip above 4096"):
+        R.validate_receipt(copy.deepcopy(r), **w)
+
+
+def test_driver_metrics_equal_oracle(oracle):
+    import numpy as np
+
+    from galaxy_amd import driver
+
+    rng = np.random.default_rng(9)
+    a = rng.normal(size=(300, 8)).astype(np.float32)
+    b = (a * (1 + 1e-3 * rng.normal(size=a.shape))).astype(np.float32)
+    b[:5, 0:2] = 0.0  # positions at the origin: the 1e-6 floor
+    got = driver.state_error(a, b)
+    w = oracle.state_error((a[:, 0], a[:, 1], a[:, 4], a[:, 5]), (b[:, 0], b[:, 1], b[:, 4], b[:, 5]))
+    for k, ok in (("position_rms_relative_l2", "position_rms"), ("position_max_relative", "position_max"),
+                  ("velocity_rms_relative_l2", "velocity_rms"), ("velocity_max_relative", "velocity_max")):
+        assert got[k] == pytest.approx(w[ok], rel=1e-12)
+    fa, fb = a[:, :4], b[:, :4]
+    rms, mx = driver.force_error(fa, fb)
+    e = oracle.force_error(fa[:, 0], fa[:, 1], fb[:, 0], fb[:, 1])
+    assert rms == pytest.approx(e["rms_relative"], rel=1e-12) and mx == e["max_relative"]
