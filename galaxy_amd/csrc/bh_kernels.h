@@ -698,7 +698,7 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
 // wavefront ballots, LDS-staged scatter, decoupled look-back across tiles).
 // Equals the stable sort_by_key of flat.rs:233-234 after four passes.
 // ----------------------------------------------------------------------------
-template <uint32_t kKPT>
+template <uint32_t kKPT, bool kLast>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __restrict__ sims,
                                                             const TileRef* __restrict__ stiles,
                                                             uint32_t nstiles, uint32_t pass,
@@ -793,17 +793,42 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
         }
     }
     __syncthreads();
-    const bool last_pass = (pass == 3);
-    for (uint32_t l = threadIdx.x; l < total; l += kSortThreads) {
-        const uint32_t kk = s_keys[l];
-        const uint32_t vv = s_vals[l];
-        const uint32_t dd = (kk >> shift) & 255u;
-        const uint32_t g = s_glob[dd] + (l - s_start[dd]);
-        kout[g] = kk;
-        vout[g] = vv;
-        if (last_pass) {  // sorted body copy
-            const float2 p = S.pos[vv];
-            S.sbody[g] = make_float4(p.x, p.y, S.mass[vv], 0.0f);
+    if (!kLast) {
+        for (uint32_t l = threadIdx.x; l < total; l += kSortThreads) {
+            const uint32_t kk = s_keys[l];
+            const uint32_t vv = s_vals[l];
+            const uint32_t dd = (kk >> shift) & 255u;
+            const uint32_t g = s_glob[dd] + (l - s_start[dd]);
+            kout[g] = kk;
+            vout[g] = vv;
+        }
+        return;
+    }
+    // Last pass: also the sorted body copy.  The body gathers of kGather keys are
+    // issued before their stores (a gather after a store to another array
+    // would wait for it).
+    constexpr uint32_t kGather = kKPT / 2 > 0 ? kKPT / 2 : 1;
+    for (uint32_t l0 = threadIdx.x; l0 < total; l0 += kGather * kSortThreads) {
+        uint32_t kk[kGather], vv[kGather], gg[kGather];
+        float2 pp[kGather];
+        float mm[kGather];
+#pragma unroll
+        for (uint32_t j = 0; j < kGather; ++j) {
+            const uint32_t l = l0 + j * kSortThreads;
+            const bool in = l < total;
+            kk[j] = in ? s_keys[l] : 0u;
+            vv[j] = in ? s_vals[l] : 0u;
+            const uint32_t dd = (kk[j] >> shift) & 255u;
+            gg[j] = in ? s_glob[dd] + (l - s_start[dd]) : 0u;
+            pp[j] = in ? S.pos[vv[j]] : make_float2(0.f, 0.f);
+            mm[j] = in ? S.mass[vv[j]] : 0.f;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kGather; ++j) {
+            if (l0 + j * kSortThreads >= total) break;
+            kout[gg[j]] = kk[j];
+            vout[gg[j]] = vv[j];
+            S.sbody[gg[j]] = make_float4(pp[j].x, pp[j].y, mm[j], 0.0f);
         }
     }
 }
@@ -828,6 +853,7 @@ constexpr uint32_t kTopoWaves = kTopoTile / 64;          // 64-position chunks o
 constexpr uint32_t kTopoThreads = 1024;                   // threads per topology workgroup
 constexpr uint32_t kTopoPer = kTopoTile / kTopoThreads;   // positions per thread (strided)
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
+constexpr uint32_t kLeafBatch = 2;           // leaf bodies whose loads are issued together
 constexpr uint32_t kNone = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint32_t p, uint32_t kp) {
@@ -842,42 +868,51 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
                                                            const TileRef* __restrict__ ttiles,
                                                            uint32_t* __restrict__ tsum) {
-    __shared__ uint32_t s_first[kLevels], s_last[kLevels];
+    // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
+    __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
     const uint32_t t = blockIdx.x;
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
-    if (threadIdx.x < kLevels) {
-        s_first[threadIdx.x] = kNone;
-        s_last[threadIdx.x] = 0u;
-    }
+    const uint32_t w = threadIdx.x >> 6;
     uint32_t dmin[kTopoPer];
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
-        dmin[j] = 17u;
-        if (p < n) {
-            dmin[j] = dmin_of(S.key0, p, S.key0[p]);
-            S.cdmin[p] = (uint8_t)dmin[j];
+        dmin[j] = p < n ? dmin_of(S.key0, p, S.key0[p]) : 17u;
+    }
+    for (uint32_t d = 0; d < kLevels; ++d) {
+        uint32_t first = kNone, last = 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < kTopoPer; ++j) {
+            const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+            const uint32_t wbase = p - lane_id();
+            const unsigned long long m = __ballot(p < n && dmin[j] <= d);
+            if (m) {
+                first = min(first, wbase + (uint32_t)__builtin_ctzll(m));
+                last = max(last, wbase + 64u - (uint32_t)__builtin_clzll(m));
+            }
+        }
+        if (lane_id() == 0) {
+            s_first[w][d] = first;
+            s_last[w][d] = last;
         }
     }
-    __syncthreads();
+    // the per-position store comes after the ballots; the barrier below waits for it
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
-        const uint32_t wbase = p - lane_id();
-        for (uint32_t d = 0; d < kLevels; ++d) {
-            const unsigned long long m = __ballot(p < n && dmin[j] <= d);
-            if (m && lane_id() == 0) {
-                atomicMin(&s_first[d], wbase + (uint32_t)__builtin_ctzll(m));
-                atomicMax(&s_last[d], wbase + 64u - (uint32_t)__builtin_clzll(m));
-            }
-        }
+        if (p < n) S.cdmin[p] = (uint8_t)dmin[j];
     }
     __syncthreads();
     if (threadIdx.x < kLevels) {
-        tsum[(size_t)t * kTopoSum + threadIdx.x] = s_first[threadIdx.x];
-        tsum[(size_t)t * kTopoSum + kLevels + threadIdx.x] = s_last[threadIdx.x];
+        uint32_t first = kNone, last = 0u;
+        for (uint32_t k = 0; k < kTopoThreads / 64; ++k) {
+            first = min(first, s_first[k][threadIdx.x]);
+            last = max(last, s_last[k][threadIdx.x]);
+        }
+        tsum[(size_t)t * kTopoSum + threadIdx.x] = first;
+        tsum[(size_t)t * kTopoSum + kLevels + threadIdx.x] = last;
     }
 }
 
@@ -902,31 +937,49 @@ __device__ __forceinline__ uint32_t wave_incl_min_rev(uint32_t v) {  // min over
 // Per simulation (one workgroup): for every tile and depth, one past the last
 // boundary before the tile (psum[t][d], 0: none) and the first boundary after it
 // (psum[t][17+d], n: none).  Wavefront w handles depths w, w + 16, ...
-__global__ __launch_bounds__(1024) void k_topo_scan(const SimDesc* __restrict__ sims,
-                                                    const uint32_t* __restrict__ tsum,
-                                                    uint32_t* __restrict__ psum) {
+__global__ __launch_bounds__(64) void k_topo_scan(const SimDesc* __restrict__ sims,
+                                                  const uint32_t* __restrict__ tsum,
+                                                  uint32_t* __restrict__ psum) {
+    // One wavefront per (simulation, depth, direction): blockIdx.x is the
+    // simulation, blockIdx.y < kLevels the forward scan of depth blockIdx.y and
+    // the rest the backward scans.  The scan walks the simulation's tiles 64 at
+    // a time; the loads of up to kScanBatch chunks are issued before their scans.
+    constexpr uint32_t kScanBatch = 8;
     const SimDesc& S = sim_at(sims, blockIdx.x);
     const uint32_t T = (S.n + kTopoTile - 1) / kTopoTile;
     const uint32_t t0 = S.first_topo_tile;
     const uint32_t lane = lane_id();
-    for (uint32_t d = threadIdx.x >> 6; d < kLevels; d += 1024 / 64) {
-        uint32_t carry = 0;  // forward: max of "last + 1" over earlier tiles
-        for (uint32_t c = 0; c < T; c += 64) {
-            const uint32_t i = c + lane;
-            const uint32_t v = i < T ? tsum[(size_t)(t0 + i) * kTopoSum + kLevels + d] : 0u;
-            const uint32_t incl = max(wave_incl_max(v), carry);
-            const uint32_t prev = (uint32_t)__shfl_up((int)incl, 1);
-            if (i < T) psum[(size_t)(t0 + i) * kTopoSum + d] = lane == 0 ? carry : prev;
-            carry = __builtin_amdgcn_readlane(incl, 63);
+    const uint32_t chunks = (T + 63) / 64;
+    const bool forward = blockIdx.y < kLevels;
+    const uint32_t d = forward ? blockIdx.y : blockIdx.y - kLevels;
+    uint32_t carry = forward ? 0u : S.n;  // max of "last + 1" before / min of "first" after
+    for (uint32_t c0 = 0; c0 < chunks; c0 += kScanBatch) {
+        const uint32_t nb = min(kScanBatch, chunks - c0);
+        uint32_t v[kScanBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k) {
+            const uint32_t c = forward ? c0 + k : chunks - 1u - (c0 + k);
+            const uint32_t i = c * 64 + lane;
+            const bool in = k < nb && i < T;
+            v[k] = forward ? (in ? tsum[(size_t)(t0 + i) * kTopoSum + kLevels + d] : 0u)
+                           : (in ? tsum[(size_t)(t0 + i) * kTopoSum + d] : kNone);
         }
-        uint32_t rcarry = S.n;  // backward: min of "first" over later tiles
-        for (uint32_t c = ((T + 63) / 64) * 64; c > 0; c -= 64) {
-            const uint32_t i = c - 64 + lane;
-            const uint32_t v = i < T ? tsum[(size_t)(t0 + i) * kTopoSum + d] : kNone;
-            const uint32_t incl = min(wave_incl_min_rev(v), rcarry);
-            const uint32_t next = (uint32_t)__shfl_down((int)incl, 1);
-            if (i < T) psum[(size_t)(t0 + i) * kTopoSum + kLevels + d] = lane == 63 ? rcarry : next;
-            rcarry = __builtin_amdgcn_readlane(incl, 0);
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k) {
+            if (k >= nb) break;  // wave-uniform
+            const uint32_t c = forward ? c0 + k : chunks - 1u - (c0 + k);
+            const uint32_t i = c * 64 + lane;
+            if (forward) {
+                const uint32_t incl = max(wave_incl_max(v[k]), carry);
+                const uint32_t prev = (uint32_t)__shfl_up((int)incl, 1);
+                if (i < T) psum[(size_t)(t0 + i) * kTopoSum + d] = lane == 0 ? carry : prev;
+                carry = __builtin_amdgcn_readlane(incl, 63);
+            } else {
+                const uint32_t incl = min(wave_incl_min_rev(v[k]), carry);
+                const uint32_t next = (uint32_t)__shfl_down((int)incl, 1);
+                if (i < T) psum[(size_t)(t0 + i) * kTopoSum + kLevels + d] = lane == 63 ? carry : next;
+                carry = __builtin_amdgcn_readlane(incl, 0);
+            }
         }
     }
 }
@@ -1007,11 +1060,16 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     __shared__ uint32_t s_pl[kTopoTile];    // start of the parent group of a column head
     __shared__ uint32_t s_mine[kTopoTile];  // exclusive (columns << 16 | cells) prefix in the tile
     __shared__ uint8_t s_dmin[kTopoTile], s_count[kTopoTile];
+    __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
     if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
     __syncthreads();
     const uint32_t t = s_ticket;
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
+    if (threadIdx.x < kLevels) s_T[threadIdx.x] = S.meta->T[threadIdx.x];
+    // the root box, loaded before any store (vmcnt counts loads and stores in
+    // order, so a load issued after stores waits for them)
+    const double root_x = S.meta->root[0], root_y = S.meta->root[1], root_h = S.meta->root[2];
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
     const uint32_t base = tr.base;
@@ -1090,7 +1148,6 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
         }
     }
     __syncthreads();
-    const SimMeta* M = S.meta;
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t o = j * kTopoThreads + threadIdx.x;
         const uint32_t p = base + o;
@@ -1113,7 +1170,7 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
         }
         const uint32_t dleaf = dmin + count - 1u;
         // geometry at dmin along the key's quadrant path (child_bounds, flat.rs:96-103)
-        double cx = M->root[0], cy = M->root[1], h = M->root[2];
+        double cx = root_x, cy = root_y, h = root_h;
         for (uint32_t l = 0; l < dmin; ++l) {
             const uint32_t q = (kp >> (30u - 2u * l)) & 3u;
             h = h * 0.5;
@@ -1129,22 +1186,35 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
             c.end = e;
             c.skip = 0u;
             c.info = d | (leaf ? kInfoLeaf : 0u);
-            c.T = M->T[d];
+            c.T = s_T[d];
             uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
             if (leaf) {
                 double mass = 0.0, wx = 0.0, wy = 0.0;
                 const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
-                for (uint32_t q = p; q < e; ++q) {
-                    const float4 bq = S.sbody[q];
-                    const double bm = bq.z;
-                    mass = mass + bm;
-                    wx = wx + bm * (double)bq.x;
-                    wy = wy + bm * (double)bq.y;
-                    const uint32_t ni = nfirst + (q - p);
-                    S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u)};
-                    S.ncold[ni] = NodeCold{q, 1u, 0.0f,
-                                           kTagBody | (q == p ? kTagFirst : 0u) | (S.val0[q] & kTagValue)};
-                    S.bnode[q] = ni;
+                // kLeafBatch bodies at a time: loads first, then the stores
+                for (uint32_t q0 = p; q0 < e; q0 += kLeafBatch) {
+                    float4 bv[kLeafBatch];
+                    uint32_t vv[kLeafBatch];
+#pragma unroll
+                    for (uint32_t j = 0; j < kLeafBatch; ++j) {
+                        const bool in = q0 + j < e;
+                        bv[j] = in ? S.sbody[q0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+                        vv[j] = in ? S.val0[q0 + j] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t j = 0; j < kLeafBatch; ++j) {
+                        const uint32_t q = q0 + j;
+                        if (q >= e) break;
+                        const float4 bq = bv[j];
+                        const double bm = bq.z;
+                        mass = mass + bm;
+                        wx = wx + bm * (double)bq.x;
+                        wy = wy + bm * (double)bq.y;
+                        const uint32_t ni = nfirst + (q - p);
+                        S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u)};
+                        S.ncold[ni] = NodeCold{q, 1u, 0.0f, kTagBody | (q == p ? kTagFirst : 0u) | (vv[j] & kTagValue)};
+                        S.bnode[q] = ni;
+                    }
                 }
                 const double comx = wx / mass, comy = wy / mass;
                 S.cagg[idx] = Agg{mass, comx, comy, 0.0};
@@ -1194,6 +1264,48 @@ __device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, ui
     return Agg{mass, wx / mass, wy / mass, 0.0};
 }
 
+// Aggregate of an internal cell lying inside its tile from its children's
+// aggregates in global memory (tiles whose cells exceed the LDS stage).
+__device__ __forceinline__ Agg combine_children_if(const SimDesc& S, uint32_t idx, bool inside) {
+    return inside ? combine_children(S.cagg, S.cchild[idx]) : Agg{0.0, 0.0, 0.0, 0.0};
+}
+
+// Records of the internal cell idx (column head position p, depth d, node ni,
+// range end e).  Inside its tile: aggregate, cell, hot node and, from LDS, child
+// links.  Crossing a tile edge: a span-list slot, the cell and node without
+// aggregates (k_agg_top fills them) and the child links held in LDS (links to
+// children in other tiles were written by those tiles).  w = 0 means the node
+// word is still to be computed from the skip target.
+__device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, uint32_t ni, uint32_t p, uint32_t d,
+                                                bool inside, const Agg& a, uint32_t e, uint32_t w,
+                                                const uint32_t* lds_ch) {
+    const float T = S.meta->T[d];
+    if (w == 0u) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d);
+    if (inside) {
+        if (lds_ch) S.cchild[idx] = make_uint4(lds_ch[0], lds_ch[1], lds_ch[2], lds_ch[3]);
+        S.cagg[idx] = a;
+        S.cells[idx] = CellT{(float)a.x, (float)a.y, (float)a.m, T, p, e, 0u, d};
+        S.nodes[ni] = NodeHot{(float)a.x, (float)a.y, (float)a.m, w};
+    } else {
+        uint32_t info = d;
+        const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
+        if (slot < S.span_cap) {
+            S.span[slot] = idx;
+            info |= (slot + 1u) << kInfoSpanShift;
+        } else {
+            atomicOr(&S.meta->err, kErrSpanCapacity);
+        }
+        // aggregates follow in k_agg_top; the cell's other fields are final
+        S.cells[idx] = CellT{0.f, 0.f, 0.f, T, p, e, 0u, info};
+        S.nodes[ni] = NodeHot{0.f, 0.f, 0.f, w};
+        if (lds_ch) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (lds_ch[q] != kNoChild) (&S.cchild[idx].x)[q] = lds_ch[q];
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------
 // K5: skip links (next cell after each subtree), the remaining child links, and
 // bottom-up aggregates of every internal cell whose range lies inside this tile.
@@ -1207,6 +1319,7 @@ __device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, ui
 // use the same rounds through global memory.
 // ----------------------------------------------------------------------------
 constexpr uint32_t kLinkCells = 1536;
+constexpr uint32_t kLinkBatch = 4;  // cells of a column whose loads are issued together
 
 __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
                                                          const TileRef* __restrict__ ttiles) {
@@ -1253,36 +1366,61 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
     uint32_t inside = 0;  // bit k: cell base + k is internal and lies inside the tile
     const uint32_t kp = ok ? S.key0[p] : 0u;
     if (ok) {
-        for (uint32_t k = 0; k < count; ++k) {
-            const uint32_t idx = base + k;
-            const uint32_t e = S.cend[idx];
-            const uint32_t d = dmin + k;
-            if (k + 1u < count) {  // internal: cold node now, the rest in its round
-                const uint32_t ni = nb + k;
-                const uint32_t skip = e < n ? S.nbase[e] : nnodes;
-                S.ncold[ni] = NodeCold{p, e - p, S.meta->T[d], d};
+        // loads that do not depend on the column's cells, issued before any store
+        uint32_t pl = 0, pl_base = 0, pl_dmin = 0;
+        if (dmin >= 1) {
+            pl = S.cparent[p];
+            pl_base = S.cbase[pl];
+            pl_dmin = S.cdmin[pl];
+        }
+        Agg leaf{0.0, 0.0, 0.0, 0.0};
+        if (in_lds) leaf = S.cagg[base + count - 1u];  // the leaf's aggregate (k_topo_count)
+        // The column's cells, kLinkBatch at a time: each batch loads its range
+        // ends, then their skip targets and thresholds, then stores.  (The
+        // compiler cannot move a load across a store to another array, so a
+        // load-store loop would pay two memory latencies per cell.)
+        for (uint32_t k0 = 0; k0 < count; k0 += kLinkBatch) {
+            uint32_t ev[kLinkBatch], sk[kLinkBatch];
+            float tv[kLinkBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kLinkBatch; ++j) ev[j] = k0 + j < count ? S.cend[base + k0 + j] : 0u;
+#pragma unroll
+            for (uint32_t j = 0; j < kLinkBatch; ++j) {
+                const bool internal = k0 + j + 1u < count;
+                sk[j] = internal ? (ev[j] < n ? S.nbase[ev[j]] : nnodes) : 0u;
+                tv[j] = internal ? S.meta->T[dmin + k0 + j] : 0.0f;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < kLinkBatch; ++j) {
+                const uint32_t k = k0 + j;
+                if (k + 1u >= count) break;
+                const uint32_t idx = base + k;
+                const uint32_t d = dmin + k;
+                const uint32_t e = ev[j];
+                const uint32_t ni = nb + k;  // internal: cold node now, the rest after the rounds
+                S.ncold[ni] = NodeCold{p, e - p, tv[j], d};
                 if (e <= tile_end) inside |= 1u << k;
                 if (in_lds) {
-                    s_w[idx - cb0] = node_word(skip - ni, d);  // internal depths are 0..15
+                    s_w[idx - cb0] = node_word(sk[j] - ni, d);  // internal depths are 0..15
                     s_e[idx - cb0] = e;
                     s_ch[idx - cb0][(kp >> (30u - 2u * d)) & 3u] = idx + 1u;
                 }
-            } else if (in_lds) {  // the leaf's aggregate (k_topo_count)
-                const Agg a = S.cagg[idx];
-                s_m[idx - cb0] = a.m;
-                s_x[idx - cb0] = a.x;
-                s_y[idx - cb0] = a.y;
             }
+        }
+        if (in_lds) {
+            const uint32_t r = base + count - 1u - cb0;
+            s_m[r] = leaf.m;
+            s_x[r] = leaf.x;
+            s_y[r] = leaf.y;
         }
         if (count > 1u) {
             atomicMin(&s_dlo, dmin);
             atomicMax(&s_dhi, dmin + count - 2u);
         }
         if (dmin >= 1) {
-            const uint32_t pl = S.cparent[p];
-            const uint32_t parent = S.cbase[pl] + (dmin - 1u - S.cdmin[pl]);
+            const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
-            if (S.cdmin[pl] <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
+            if (pl_dmin <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
                 if (in_lds && parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = base;
                 else (&S.cchild[parent].x)[q] = base;  // a parent outside the tile's LDS range
             } else {
@@ -1292,29 +1430,19 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
     }
     __syncthreads();
     const uint32_t dlo = s_dlo, dhi = s_dhi;
+    // Bottom-up rounds.  With the tile's cells in LDS a round is LDS traffic
+    // only: a barrier also waits for the wavefront's outstanding global stores,
+    // so the records are written after the last round, each cell once.
     for (int d = (int)dhi; d >= (int)dlo; --d) {
         const uint32_t k = (uint32_t)d - dmin;
         if (count > 1u && (uint32_t)d >= dmin && k + 1u < count) {
             const uint32_t idx = base + k;
-            const uint32_t ni = nb + k;
-            uint32_t e, w;
             if (in_lds) {
-                e = s_e[idx - cb0];
-                w = s_w[idx - cb0];
-            } else {
-                e = S.cend[idx];
-                w = node_word((e < n ? S.nbase[e] : nnodes) - ni, (uint32_t)d);
-            }
-            const float T = S.meta->T[d];
-            if (inside & (1u << k)) {
-                Agg a;
-                if (in_lds) {
+                if (inside & (1u << k)) {
                     double mass = 0.0, wx = 0.0, wy = 0.0;
-                    uint4 ch;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t c = s_ch[idx - cb0][q];
-                        (&ch.x)[q] = c;
                         if (c != kNoChild) {
                             const double cm = s_m[c - cb0];
                             mass = mass + cm;
@@ -1322,39 +1450,26 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                             wy = wy + cm * s_y[c - cb0];
                         }
                     }
-                    a = Agg{mass, wx / mass, wy / mass, 0.0};
-                    s_m[idx - cb0] = a.m;
-                    s_x[idx - cb0] = a.x;
-                    s_y[idx - cb0] = a.y;
-                    S.cchild[idx] = ch;
-                } else {
-                    a = combine_children(S.cagg, S.cchild[idx]);
+                    s_m[idx - cb0] = mass;
+                    s_x[idx - cb0] = wx / mass;
+                    s_y[idx - cb0] = wy / mass;
                 }
-                S.cagg[idx] = a;
-                S.cells[idx] = CellT{(float)a.x, (float)a.y, (float)a.m, T, p, e, 0u, (uint32_t)d};
-                S.nodes[ni] = NodeHot{(float)a.x, (float)a.y, (float)a.m, w};
             } else {
-                uint32_t info = (uint32_t)d;
-                const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
-                if (slot < S.span_cap) {
-                    S.span[slot] = idx;
-                    info |= (slot + 1u) << kInfoSpanShift;
-                } else {
-                    atomicOr(&S.meta->err, kErrSpanCapacity);
-                }
-                // aggregates follow in k_agg_top; the cell's other fields are final
-                S.cells[idx] = CellT{0.f, 0.f, 0.f, T, p, e, 0u, info};
-                S.nodes[ni] = NodeHot{0.f, 0.f, 0.f, w};
-                if (in_lds) {  // children inside the tile; others were written directly
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint32_t c = s_ch[idx - cb0][q];
-                        if (c != kNoChild) (&S.cchild[idx].x)[q] = c;
-                    }
-                }
+                link_write_cell(S, idx, nb + k, p, (uint32_t)d, (inside & (1u << k)) != 0u,
+                                combine_children_if(S, idx, (inside & (1u << k)) != 0u), S.cend[idx],
+                                0u, nullptr);
             }
         }
         __syncthreads();
+    }
+    if (in_lds) {
+        for (uint32_t k = 0; k + 1u < count; ++k) {
+            const uint32_t idx = base + k;
+            const uint32_t r = idx - cb0;
+            const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
+            link_write_cell(S, idx, nb + k, p, dmin + k, (inside & (1u << k)) != 0u, a, s_e[r], s_w[r],
+                            s_ch[r]);
+        }
     }
 }
 
@@ -1371,15 +1486,24 @@ constexpr uint32_t kTopThreads = 1024;
 constexpr uint32_t kTopLds = 4096;  // aggregate slots: spanning cells, then preloaded children
 
 __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims) {
-    __shared__ double s_m[kTopLds], s_x[kTopLds], s_y[kTopLds];
+    // In LDS mode the aggregates live in s_mxy (m, x, y planes); otherwise the
+    // same memory holds the spanning cells bucketed by depth (kTopListCap slots).
+    __shared__ double s_mxy[3 * kTopLds];
     __shared__ uint32_t s_ref[kTopLds / 5][4];
     __shared__ uint8_t s_depth[kTopLds / 5];
-    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
+    double* s_m = s_mxy;
+    double* s_x = s_mxy + kTopLds;
+    double* s_y = s_mxy + 2 * kTopLds;
+    uint32_t* s_list = (uint32_t*)s_mxy;
+    constexpr uint32_t kTopListCap = 3u * kTopLds * 2u;
     const SimDesc& S = sim_at(sims, blockIdx.x);
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
     const bool in_lds = nspan <= kTopLds / 5;
+    const bool listed = !in_lds && nspan <= kTopListCap;
     if (threadIdx.x == 0) s_next = nspan;
+    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
     if (in_lds) {
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
@@ -1406,9 +1530,32 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             }
         }
         __syncthreads();
+    } else if (listed) {
+        // counting sort of the spanning cells by depth, so that every depth
+        // round below visits only its own cells
+        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
+            atomicAdd(&s_cnt[min(S.cells[S.span[i]].info & 255u, kLevels - 1u)], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (uint32_t d = 0; d < kLevels; ++d) {
+                s_off[d] = acc;
+                acc += s_cnt[d];
+                s_cnt[d] = 0u;
+            }
+            s_off[kLevels] = acc;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+            const uint32_t d = min(S.cells[S.span[i]].info & 255u, kLevels - 1u);
+            s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = i;
+        }
+        __syncthreads();
     }
     for (int d = 15; d >= 0; --d) {
-        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+        const uint32_t lo = listed ? s_off[d] : 0u, hi = listed ? s_off[d + 1] : nspan;
+        for (uint32_t k = lo + threadIdx.x; k < hi; k += kTopThreads) {
+            const uint32_t i = listed ? s_list[k] : k;
             const uint32_t idx = S.span[i];
             const uint32_t depth = in_lds ? s_depth[i] : (S.cells[idx].info & 255u);
             if (depth != (uint32_t)d) continue;

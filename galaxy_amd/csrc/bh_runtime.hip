@@ -268,15 +268,18 @@ struct SimSet {
         HIP_OK(hipGetLastError());
     }
     void sort(hipStream_t s) {
+        // the last pass (which also writes the sorted body copy) is its own
+        // instance, so its registers do not limit the occupancy of the others
         for (uint32_t pass = 0; pass < 4; ++pass) {
-            if (sort_kpt == 8)
-                hipLaunchKernelGGL(k_sort_pass<8>, dim3((uint32_t)tsort.size()), dim3(kSortThreads),
-                                   0, s, d_desc, d_tsort, (uint32_t)tsort.size(), pass, d_ctl,
-                                   d_sflags);
-            else
-                hipLaunchKernelGGL(k_sort_pass<4>, dim3((uint32_t)tsort.size()), dim3(kSortThreads),
-                                   0, s, d_desc, d_tsort, (uint32_t)tsort.size(), pass, d_ctl,
-                                   d_sflags);
+            const dim3 grid((uint32_t)tsort.size()), block(kSortThreads);
+            const uint32_t nt = (uint32_t)tsort.size();
+            if (sort_kpt == 8) {
+                if (pass < 3) hipLaunchKernelGGL((k_sort_pass<8, false>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
+                else hipLaunchKernelGGL((k_sort_pass<8, true>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
+            } else {
+                if (pass < 3) hipLaunchKernelGGL((k_sort_pass<4, false>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
+                else hipLaunchKernelGGL((k_sort_pass<4, true>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
+            }
             HIP_OK(hipGetLastError());
         }
     }
@@ -284,7 +287,7 @@ struct SimSet {
         hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
                            d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size()), dim3(1024), 0, s, d_desc, d_tsum,
+        hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size(), 2 * kLevels), dim3(64), 0, s, d_desc, d_tsum,
                            d_psum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_count, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
