@@ -137,7 +137,6 @@ struct __attribute__((aligned(16))) SimDesc {
     // cells (preorder)
     CellT* cells;
     uint32_t* cend;    // end of each cell's range (written by k_topo_count)
-    float4* cgeom;     // f32 (cx, cy, half, 0)
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
     uint32_t* span;    // spanning internal cells (span_cap entries)
@@ -853,7 +852,7 @@ constexpr uint32_t kTopoWaves = kTopoTile / 64;          // 64-position chunks o
 constexpr uint32_t kTopoThreads = 1024;                   // threads per topology workgroup
 constexpr uint32_t kTopoPer = kTopoTile / kTopoThreads;   // positions per thread (strided)
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
-constexpr uint32_t kLeafBatch = 2;           // leaf bodies whose loads are issued together
+constexpr uint32_t kLeafBatch = 4;           // leaf bodies whose loads are issued together
 constexpr uint32_t kNone = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint32_t p, uint32_t kp) {
@@ -991,9 +990,18 @@ struct TileBounds {
     uint32_t after[kTopoWaves][kLevels];           // first boundary in later waves (kNone: none)
     uint32_t pre[kLevels], post[kLevels];          // tile summaries of the other tiles
 
+    // The other tiles' summaries; called before the barrier that precedes build.
+    __device__ void load_summaries(const uint32_t* __restrict__ ps) {
+        if (threadIdx.x < kLevels) {
+            pre[threadIdx.x] = ps[threadIdx.x];
+            post[threadIdx.x] = ps[kLevels + threadIdx.x];
+        }
+    }
+
     // Every thread of the workgroup calls this (block-wide barriers), after a
-    // barrier that publishes dmin[] (LDS, one byte per position of the tile).
-    __device__ void build(uint32_t base, uint32_t n, const uint8_t* dmin, const uint32_t* __restrict__ ps) {
+    // barrier that publishes dmin[] (LDS, one byte per position of the tile)
+    // and the summaries of load_summaries.
+    __device__ void build(uint32_t base, uint32_t n, const uint8_t* dmin) {
         for (uint32_t c = threadIdx.x >> 6; c < kTopoWaves; c += blockDim.x >> 6) {
             const uint32_t o = c * 64u + lane_id();
             const uint32_t dm = base + o < n ? dmin[o] : 17u;
@@ -1001,10 +1009,6 @@ struct TileBounds {
                 const unsigned long long m = __ballot(dm <= d);
                 if (lane_id() == 0) mask[c][d] = m;
             }
-        }
-        if (threadIdx.x < kLevels) {
-            pre[threadIdx.x] = ps[threadIdx.x];
-            post[threadIdx.x] = ps[kLevels + threadIdx.x];
         }
         __syncthreads();
         if (threadIdx.x < kLevels) {
@@ -1067,9 +1071,6 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
     if (threadIdx.x < kLevels) s_T[threadIdx.x] = S.meta->T[threadIdx.x];
-    // the root box, loaded before any store (vmcnt counts loads and stores in
-    // order, so a load issued after stores waits for them)
-    const double root_x = S.meta->root[0], root_y = S.meta->root[1], root_h = S.meta->root[2];
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
     const uint32_t base = tr.base;
@@ -1081,8 +1082,9 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
         s_key[o] = p < n ? K[p] : 0u;
         s_dmin[o] = p < n ? S.cdmin[p] : (uint8_t)17u;
     }
+    tb.load_summaries(psum + (size_t)t * kTopoSum);
     __syncthreads();
-    tb.build(base, n, s_dmin, psum + (size_t)t * kTopoSum);
+    tb.build(base, n, s_dmin);
     auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
     // columns: existence from the parent group's size, leaf depth from the key at p + bucket
 #pragma unroll
@@ -1169,14 +1171,8 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
             continue;
         }
         const uint32_t dleaf = dmin + count - 1u;
-        // geometry at dmin along the key's quadrant path (child_bounds, flat.rs:96-103)
-        double cx = root_x, cy = root_y, h = root_h;
-        for (uint32_t l = 0; l < dmin; ++l) {
-            const uint32_t q = (kp >> (30u - 2u * l)) & 3u;
-            h = h * 0.5;
-            cx = cx + ((q & 1u) ? h : -h);
-            cy = cy + ((q & 2u) ? h : -h);
-        }
+        // (cell geometry is not stored: evidence readbacks derive it from the
+        // cell's start key and depth, as child_bounds does, flat.rs:96-103)
         for (uint32_t d = dmin; d <= dleaf; ++d) {
             const uint32_t idx = cbase + (d - dmin);
             const uint32_t e = tb.next(base, p, d);
@@ -1194,13 +1190,9 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
                 // kLeafBatch bodies at a time: loads first, then the stores
                 for (uint32_t q0 = p; q0 < e; q0 += kLeafBatch) {
                     float4 bv[kLeafBatch];
-                    uint32_t vv[kLeafBatch];
 #pragma unroll
-                    for (uint32_t j = 0; j < kLeafBatch; ++j) {
-                        const bool in = q0 + j < e;
-                        bv[j] = in ? S.sbody[q0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
-                        vv[j] = in ? S.val0[q0 + j] : 0u;
-                    }
+                    for (uint32_t j = 0; j < kLeafBatch; ++j)
+                        bv[j] = q0 + j < e ? S.sbody[q0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                     for (uint32_t j = 0; j < kLeafBatch; ++j) {
                         const uint32_t q = q0 + j;
@@ -1212,7 +1204,6 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
                         wy = wy + bm * (double)bq.y;
                         const uint32_t ni = nfirst + (q - p);
                         S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u)};
-                        S.ncold[ni] = NodeCold{q, 1u, 0.0f, kTagBody | (q == p ? kTagFirst : 0u) | (vv[j] & kTagValue)};
                         S.bnode[q] = ni;
                     }
                 }
@@ -1232,16 +1223,14 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
                 c.mass = 0.f;
             }
             // internal cells' records are written whole by k_topo_link once their
-            // aggregates are known; here only their range end and first child
-            if (leaf) S.cells[idx] = c;
-            S.cend[idx] = e;
-            S.cchild[idx] = ch;
-            S.cgeom[idx] = make_float4((float)cx, (float)cy, (float)h, 0.0f);
-            if (!leaf) {
-                const uint32_t q = (kp >> (30u - 2u * d)) & 3u;
-                h = h * 0.5;
-                cx = cx + ((q & 1u) ? h : -h);
-                cy = cy + ((q & 2u) ? h : -h);
+            // aggregates are known; here only their range end and first child.
+            // A leaf's range end is in its record and it has no children
+            // (readbacks supply the empty links).
+            if (leaf) {
+                S.cells[idx] = c;
+            } else {
+                S.cend[idx] = e;
+                S.cchild[idx] = ch;
             }
         }
     }
@@ -1383,7 +1372,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             uint32_t ev[kLinkBatch], sk[kLinkBatch];
             float tv[kLinkBatch];
 #pragma unroll
-            for (uint32_t j = 0; j < kLinkBatch; ++j) ev[j] = k0 + j < count ? S.cend[base + k0 + j] : 0u;
+            for (uint32_t j = 0; j < kLinkBatch; ++j) ev[j] = k0 + j + 1u < count ? S.cend[base + k0 + j] : 0u;
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
                 const bool internal = k0 + j + 1u < count;
@@ -1617,16 +1606,36 @@ __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
     if (threadIdx.x == 0) S.meta->theta_bits = __float_as_uint(S.theta);
 }
 
+// Thresholds of a device-built tree for a new theta: every cell, and the cold
+// record of every internal cell's node (the node of the cell at depth d of the
+// column at position p is nbase[p] + d - cdmin[p]).
 __global__ void k_rewrite_T(const SimDesc* __restrict__ sims, uint32_t sim) {
     const SimDesc& S = sim_at(sims, sim);
-    const uint32_t nc = S.meta->ncells;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
-        S.cells[i].T = S.meta->T[S.cells[i].info & 255u];
-    const uint32_t nn = S.meta->nnodes;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) {
-        const uint32_t tag = S.ncold[i].tag;
-        if (!(tag & kTagBody)) S.ncold[i].T = S.meta->T[tag & 255u];
+    const uint32_t nc = min(S.meta->ncells, S.cell_cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const CellT c = S.cells[i];
+        const uint32_t d = c.info & 255u;
+        const float T = S.meta->T[d < kLevels ? d : kLevels - 1u];
+        S.cells[i].T = T;
+        if (!(c.info & kInfoLeaf) && c.start < S.n) {
+            const uint32_t ni = S.nbase[c.start] + (d - S.cdmin[c.start]);
+            if (ni < S.node_cap) S.ncold[ni].T = T;
+        }
     }
+}
+
+// Cold records of the body nodes of device-built trees, which the step does not
+// write (only the evidence traversal reads them): body at sorted position p,
+// first of its leaf iff a column of cells starts at p.
+__global__ __launch_bounds__(kTile) void k_fill_body_cold(const SimDesc* __restrict__ sims,
+                                                          const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    const SimDesc& S = sim_at(sims, tr.sim);
+    const uint32_t p = tr.base + threadIdx.x;
+    if (p >= S.n) return;
+    const uint32_t ni = S.bnode[p];
+    if (ni < S.node_cap)
+        S.ncold[ni] = NodeCold{p, 1u, 0.0f, kTagBody | (S.ccount[p] > 0u ? kTagFirst : 0u) | (S.val0[p] & kTagValue)};
 }
 
 // ----------------------------------------------------------------------------

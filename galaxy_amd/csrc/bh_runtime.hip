@@ -106,7 +106,6 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.ccount = (uint8_t*)take(1ull * n);
     d.cells = (CellT*)take(sizeof(CellT) * cap);
     d.cend = (uint32_t*)take(4ull * cap);
-    d.cgeom = (float4*)take(16ull * cap);
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
     d.span = (uint32_t*)take(4ull * scap);
@@ -322,6 +321,11 @@ struct SimSet {
                                0xffffffffu);
         HIP_OK(hipGetLastError());
     }
+    // body-node cold records for an evidence traversal (the step leaves them unwritten)
+    void fill_body_cold(hipStream_t s) {
+        hipLaunchKernelGGL(k_fill_body_cold, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
+        HIP_OK(hipGetLastError());
+    }
     void final_kick(hipStream_t s) {
         hipLaunchKernelGGL(k_final_kick, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
         HIP_OK(hipGetLastError());
@@ -483,7 +487,6 @@ void read_tree(const SimDesc& d, TreeHost& h, hipStream_t s) {
     h.val.resize(d.n);
     h.pos_of.resize(d.n);
     HIP_OK(hipMemcpyAsync(h.cells.data(), d.cells, sizeof(CellT) * h.ncells, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(h.geom.data(), d.cgeom, 16ull * h.ncells, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(h.child.data(), d.cchild, 16ull * h.ncells, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(h.key.data(), d.key0, 4ull * d.n, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(h.val.data(), d.val0, 4ull * d.n, hipMemcpyDeviceToHost, s));
@@ -492,6 +495,23 @@ void read_tree(const SimDesc& d, TreeHost& h, hipStream_t s) {
     for (uint32_t p = 0; p < d.n; ++p) {
         if (h.val[p] >= d.n) throw HipError{"tree has an invalid sort permutation"};
         h.pos_of[h.val[p]] = p;
+    }
+    // cell geometry: the f64 root box halved along the quadrant path of the
+    // cell's start key to its depth (child_bounds, flat.rs:96-103), rounded once
+    for (uint32_t k = 0; k < h.ncells; ++k) {
+        const CellT& c = h.cells[k];
+        const uint32_t depth = c.info & 255u;
+        if (c.info & kInfoLeaf) h.child[k] = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);  // not stored
+        if (c.start >= d.n || depth >= kLevels) throw HipError{"tree has an invalid cell"};
+        const uint32_t key = h.key[c.start];
+        double cx = h.meta.root[0], cy = h.meta.root[1], hh = h.meta.root[2];
+        for (uint32_t l = 0; l < depth; ++l) {
+            const uint32_t q = (key >> (30u - 2u * l)) & 3u;
+            hh = hh * 0.5;
+            cx = cx + ((q & 1u) ? hh : -hh);
+            cy = cy + ((q & 2u) ? hh : -hh);
+        }
+        h.geom[k] = make_float4((float)cx, (float)cy, (float)hh, 0.0f);
     }
 }
 
@@ -927,6 +947,7 @@ static int force_impl(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_con
         unsigned long long* d_hash = nullptr;
         if (stats) HIP_OK(hipMalloc((void**)&d_stats, 12ull * t->n));
         if (hash) HIP_OK(hipMalloc((void**)&d_hash, 8ull * t->n));
+        if (d_stats || d_hash) t->set.fill_body_cold(s);
         HIP_OK(hipEventRecord(t->ev[6], s));
         t->set.force(s, true, d_stats, d_hash);
         HIP_OK(hipEventRecord(t->ev[7], s));
