@@ -1267,7 +1267,7 @@ __device__ __forceinline__ Agg combine_children_if(const SimDesc& S, uint32_t id
 // word is still to be computed from the skip target.
 __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, uint32_t ni, uint32_t p, uint32_t d,
                                                 bool inside, const Agg& a, uint32_t e, uint32_t w,
-                                                const uint32_t* lds_ch) {
+                                                const uint32_t* lds_ch, uint32_t slot = kNone) {
     const float T = S.meta->T[d];
     if (w == 0u) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d);
     if (inside) {
@@ -1277,7 +1277,7 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
         S.nodes[ni] = NodeHot{(float)a.x, (float)a.y, (float)a.m, w};
     } else {
         uint32_t info = d;
-        const uint32_t slot = atomicAdd(&S.meta->nspan, 1u);
+        if (slot == kNone) slot = atomicAdd(&S.meta->nspan, 1u);
         if (slot < S.span_cap) {
             S.span[slot] = idx;
             info |= (slot + 1u) << kInfoSpanShift;
@@ -1402,10 +1402,6 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             s_x[r] = leaf.x;
             s_y[r] = leaf.y;
         }
-        if (count > 1u) {
-            atomicMin(&s_dlo, dmin);
-            atomicMax(&s_dhi, dmin + count - 2u);
-        }
         if (dmin >= 1) {
             const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
@@ -1416,6 +1412,14 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
                 atomicOr(&S.meta->err, kErrCellCapacity);
             }
         }
+    }
+    // depth range of the tile's internal cells: reduced per wavefront first (an
+    // LDS atomic per thread would serialize on two words)
+    {
+        const uint32_t lo = wave_incl_min_rev(count > 1u ? dmin : 16u);
+        const uint32_t hi = wave_incl_max(count > 1u ? dmin + count - 2u : 0u);
+        if (lane_id() == 0) atomicMin(&s_dlo, lo);
+        if (lane_id() == 63) atomicMax(&s_dhi, hi);
     }
     __syncthreads();
     const uint32_t dlo = s_dlo, dhi = s_dhi;
@@ -1451,13 +1455,17 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
         }
         __syncthreads();
     }
-    if (in_lds) {
+    if (in_lds && count > 1u) {
+        // the column's tile-crossing cells take consecutive span slots, one atomic per column
+        const uint32_t crossing = ((1u << (count - 1u)) - 1u) & ~inside;
+        uint32_t slot = crossing ? atomicAdd(&S.meta->nspan, (uint32_t)__builtin_popcount(crossing)) : 0u;
         for (uint32_t k = 0; k + 1u < count; ++k) {
             const uint32_t idx = base + k;
             const uint32_t r = idx - cb0;
             const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
-            link_write_cell(S, idx, nb + k, p, dmin + k, (inside & (1u << k)) != 0u, a, s_e[r], s_w[r],
-                            s_ch[r]);
+            const bool in = (inside & (1u << k)) != 0u;
+            link_write_cell(S, idx, nb + k, p, dmin + k, in, a, s_e[r], s_w[r], s_ch[r], in ? kNone : slot);
+            if (!in) ++slot;
         }
     }
 }
@@ -1502,13 +1510,18 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t c = (&ch.x)[q];
-                uint32_t r = kNoChild;
-                if (c != kNoChild) {
-                    const uint32_t slot = S.cells[c].info >> kInfoSpanShift;
-                    if (slot != 0u) {
-                        r = slot - 1u;
-                    } else {
-                        r = atomicAdd(&s_next, 1u);
+                const uint32_t slot = c != kNoChild ? S.cells[c].info >> kInfoSpanShift : 0u;
+                const bool preload = c != kNoChild && slot == 0u;
+                // preload slots: one LDS atomic per wavefront, ranks from the ballot
+                const unsigned long long m = __ballot(preload);
+                uint32_t r = c == kNoChild ? kNoChild : slot - 1u;
+                if (m) {
+                    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+                    uint32_t first = 0u;
+                    if (lane_id() == leader) first = atomicAdd(&s_next, (uint32_t)__builtin_popcountll(m));
+                    first = __shfl(first, (int)leader);
+                    if (preload) {
+                        r = first + (uint32_t)__builtin_popcountll(m & ((1ull << lane_id()) - 1ull));
                         const Agg a = S.cagg[c];
                         s_m[r] = a.m;
                         s_x[r] = a.x;
