@@ -3,13 +3,19 @@
 
 One JSON line on stdout (rank 0, one GPU):
   value          batched ensemble throughput (body-steps/s): 256 rotating discs,
-                 N log-uniform in [512, 65536], one launch per phase, HIP-graph steps
+                 N log-uniform in [512, 65536], one launch per phase, HIP-graph steps,
+                 at the launch shape the autotuner picks (config.batched_choice; the
+                 tuner's measurements in autotune; --choice or --no-tune to fix it)
   plain_*        the drop-in at N=65,536 (cold collapse), driven step by step through
                  the synchronous C ABI exactly like galaxy-bh-gpu-tree-parallel.rs:345-350
+  large_single   one 1,048,576-body binary disc encounter (theta 0.6) through the
+                 beyond-cap drop-in tree
   serving        a bounded sample of the request stream: p50/p99 latency, body-steps/s
-  roofline       the traversal kernel (dominant) of the batched step, FP32 flops
+  roofline       the traversal kernel (dominant) of the batched step, FP32 flops; HBM
+                 traffic from profiles/traverse_batched256_traffic.json
   cpu_baseline   the oracle's f64 BH #2A CPU trajectory (cpu_flat_evolve restatement),
                  single thread, on a bounded prefix of the N=65,536 run
+--only plain|batched|large|copy runs one leg (for profiling) and prints its result.
 A step = kick/drift + full tree rebuild + force traversal + final kick, inputs resident in HBM.
 """
 from __future__ import annotations
