@@ -32,8 +32,8 @@ constexpr uint32_t kSpanSlotsLds = 4096; // spanning cells aggregated in LDS
 
 // Control words shared by the kernels of one simulation set.
 enum : uint32_t {
-    kCtlSortTicket = 0,  // 4 words, one per radix pass
-    kCtlTopoTicket = 4,
+    kCtlSortTicket = 0,  // unused (tile tickets are per simulation, SimMeta::ticket)
+    kCtlTopoTicket = 4,  // unused
     kCtlQueue = 5,
     kCtlError = 6,
     kCtlWords = 16,
@@ -109,6 +109,8 @@ struct SimMeta {
     uint32_t err;
     uint32_t theta_bits;    // f32 theta the thresholds were computed for
     uint32_t hist[4 * 256]; // per-pass digit histograms of the Morton keys
+    // tile tickets of the simulation: radix passes 0..3, topology (4)
+    uint32_t ticket[8];
 };
 
 struct __attribute__((aligned(16))) SimDesc {
@@ -508,6 +510,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0ull;
     if (tr.base == 0) {
         for (uint32_t k = threadIdx.x; k < 4 * 256; k += kTile) S.meta->hist[k] = 0u;
+        if (threadIdx.x < 8) S.meta->ticket[threadIdx.x] = 0u;
     }
     float x = 0.f, y = 0.f;
     if (valid) {
@@ -592,12 +595,14 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
     const TileRef tr = stiles[blockIdx.x];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
-    // reset this tile's look-back flags for the four radix passes
-    for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
-        uint32_t pass = k / kRadix, d = k % kRadix;
-        sflags[((size_t)pass * nstiles + blockIdx.x) * kRadix + d] = 0u;
-        s_hist[pass][d] = 0u;
-    }
+    // Every load of the kernel is issued before its first store: the positions
+    // of this tile, then the bounds partials (a load issued after a store
+    // waits for the store).
+    // (Unconditional loads at a clamped index: a conditional load would be
+    // waited for inside its branch.)
+    float2 pos[kKPT];
+#pragma unroll
+    for (uint32_t j = 0; j < kKPT; ++j) pos[j] = S.pos[min(tr.base + j * kSortThreads + threadIdx.x, n - 1u)];
     // global f32 min/max of the simulation from the bounds partials
     const uint32_t nparts = (n + kTile - 1) / kTile;
     float mnx = __builtin_inff(), mny = __builtin_inff(), mxx = -__builtin_inff(), mxy = -__builtin_inff();
@@ -607,6 +612,12 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
         mny = fminf(mny, b.y);
         mxx = fmaxf(mxx, b.z);
         mxy = fmaxf(mxy, b.w);
+    }
+    // reset this tile's look-back flags for the four radix passes
+    for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
+        uint32_t pass = k / kRadix, d = k % kRadix;
+        sflags[((size_t)pass * nstiles + blockIdx.x) * kRadix + d] = 0u;
+        s_hist[pass][d] = 0u;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -673,7 +684,7 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
     for (uint32_t j = 0; j < kKPT; ++j) {
         uint32_t i = tr.base + j * kSortThreads + threadIdx.x;
         if (i < n) {
-            float2 p = S.pos[i];
+            const float2 p = pos[j];
             uint32_t code = spread16(quantize_axis((double)p.x, cx, half)) |
                             (spread16(quantize_axis((double)p.y, cy, half)) << 1);
             S.key0[i] = code;
@@ -697,7 +708,7 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
 // wavefront ballots, LDS-staged scatter, decoupled look-back across tiles).
 // Equals the stable sort_by_key of flat.rs:233-234 after four passes.
 // ----------------------------------------------------------------------------
-template <uint32_t kKPT, bool kLast>
+template <uint32_t kKPT>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __restrict__ sims,
                                                             const TileRef* __restrict__ stiles,
                                                             uint32_t nstiles, uint32_t pass,
@@ -714,12 +725,22 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     __shared__ uint32_t s_scan[kWaves];
     __shared__ uint32_t s_ticket;
 
-    if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlSortTicket + pass], 1u);
+    // The workgroup's simulation comes from its block index; its tile within
+    // the simulation from a ticket of that simulation, so tiles take tickets
+    // in order and a look-back only waits for tiles whose workgroups run.
+    // (One ticket counter per simulation: a single counter for every tile
+    // serialises on one address.)
+    const uint32_t sim = stiles[blockIdx.x].sim;
+    const SimDesc& S = sim_at(sims, sim);
+    if (threadIdx.x == 0) s_ticket = S.first_sort_tile + atomicAdd(&S.meta->ticket[pass], 1u);
     for (uint32_t k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&s_wcnt[0][0])[k] = 0u;
     __syncthreads();
-    const uint32_t t = s_ticket;
+    const uint32_t t = min(s_ticket, nstiles - 1u);
     const TileRef tr = stiles[t];
-    const SimDesc& S = sim_at(sims, tr.sim);
+    if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
+        if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
+        return;
+    }
     const uint32_t n = S.n;
     const uint32_t shift = 8u * pass;
     const uint32_t* __restrict__ kin = (pass & 1u) ? S.key1 : S.key0;
@@ -732,12 +753,19 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     const uint32_t lane = lane_id();
     const unsigned long long lt_mask = __lanemask_lt();
     uint32_t key[kKPT], val[kKPT], rank[kKPT];
+    // all loads first: the ranking below is ordered by wavefront barriers,
+    // which a load issued inside the loop could not pass
 #pragma unroll
     for (uint32_t j = 0; j < kKPT; ++j) {
         const uint32_t i = tr.base + w * kChunk + j * 64 + lane;
         const bool valid = i < n;
         key[j] = valid ? kin[i] : 0u;
         val[j] = valid ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kKPT; ++j) {
+        const uint32_t i = tr.base + w * kChunk + j * 64 + lane;
+        const bool valid = i < n;
         const uint32_t d = (key[j] >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -792,44 +820,30 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
         }
     }
     __syncthreads();
-    if (!kLast) {
-        for (uint32_t l = threadIdx.x; l < total; l += kSortThreads) {
-            const uint32_t kk = s_keys[l];
-            const uint32_t vv = s_vals[l];
-            const uint32_t dd = (kk >> shift) & 255u;
-            const uint32_t g = s_glob[dd] + (l - s_start[dd]);
-            kout[g] = kk;
-            vout[g] = vv;
-        }
-        return;
+    for (uint32_t l = threadIdx.x; l < total; l += kSortThreads) {
+        const uint32_t kk = s_keys[l];
+        const uint32_t vv = s_vals[l];
+        const uint32_t dd = (kk >> shift) & 255u;
+        const uint32_t g = s_glob[dd] + (l - s_start[dd]);
+        kout[g] = kk;
+        vout[g] = vv;
     }
-    // Last pass: also the sorted body copy.  The body gathers of kGather keys are
-    // issued before their stores (a gather after a store to another array
-    // would wait for it).
-    constexpr uint32_t kGather = kKPT / 2 > 0 ? kKPT / 2 : 1;
-    for (uint32_t l0 = threadIdx.x; l0 < total; l0 += kGather * kSortThreads) {
-        uint32_t kk[kGather], vv[kGather], gg[kGather];
-        float2 pp[kGather];
-        float mm[kGather];
-#pragma unroll
-        for (uint32_t j = 0; j < kGather; ++j) {
-            const uint32_t l = l0 + j * kSortThreads;
-            const bool in = l < total;
-            kk[j] = in ? s_keys[l] : 0u;
-            vv[j] = in ? s_vals[l] : 0u;
-            const uint32_t dd = (kk[j] >> shift) & 255u;
-            gg[j] = in ? s_glob[dd] + (l - s_start[dd]) : 0u;
-            pp[j] = in ? S.pos[vv[j]] : make_float2(0.f, 0.f);
-            mm[j] = in ? S.mass[vv[j]] : 0.f;
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kGather; ++j) {
-            if (l0 + j * kSortThreads >= total) break;
-            kout[gg[j]] = kk[j];
-            vout[gg[j]] = vv[j];
-            S.sbody[gg[j]] = make_float4(pp[j].x, pp[j].y, mm[j], 0.0f);
-        }
-    }
+}
+
+// Sorted body copy after the last radix pass: sbody[p] = (x, y, m) of the body
+// at sorted position p.  A kernel of its own: its reads go to random body
+// indices, and inside the last pass their latency held every tile's
+// workgroup (and the look-back of the tiles after it).
+__global__ __launch_bounds__(kTile) void k_gather_sorted(const SimDesc* __restrict__ sims,
+                                                         const TileRef* __restrict__ tiles) {
+    const TileRef tr = tiles[blockIdx.x];
+    const SimDesc& S = sim_at(sims, tr.sim);
+    const uint32_t p = tr.base + threadIdx.x;
+    if (p >= S.n) return;
+    const uint32_t v = S.val0[p];
+    const float2 q = S.pos[v];
+    const float m = S.mass[v];
+    S.sbody[p] = make_float4(q.x, q.y, m, 0.0f);
 }
 
 // ----------------------------------------------------------------------------
@@ -1065,11 +1079,18 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     __shared__ uint32_t s_mine[kTopoTile];  // exclusive (columns << 16 | cells) prefix in the tile
     __shared__ uint8_t s_dmin[kTopoTile], s_count[kTopoTile];
     __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
-    if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl[kCtlTopoTicket], 1u);
+    // simulation from the block index, tile from a ticket of that simulation
+    // (as in k_sort_pass)
+    const uint32_t sim = ttiles[blockIdx.x].sim;
+    const SimDesc& S = sim_at(sims, sim);
+    if (threadIdx.x == 0) s_ticket = S.first_topo_tile + atomicAdd(&S.meta->ticket[4], 1u);
     __syncthreads();
-    const uint32_t t = s_ticket;
+    const uint32_t t = min(s_ticket, (uint32_t)(gridDim.x - 1u));
     const TileRef tr = ttiles[t];
-    const SimDesc& S = sim_at(sims, tr.sim);
+    if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
+        if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
+        return;
+    }
     if (threadIdx.x < kLevels) s_T[threadIdx.x] = S.meta->T[threadIdx.x];
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
@@ -1766,10 +1787,13 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
     const uint32_t tab = __builtin_amdgcn_readfirstlane(
         (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)tabp);
     const uint32_t lane = lane_id();
+    // The first item of every wavefront is its index in the grid; later items
+    // come from the queue counter, offset by the grid's wavefront count (so
+    // the launch does not start with every wavefront on one atomic).
+    const uint32_t grid_waves = gridDim.x * (blockDim.x >> 6);
+    uint32_t next_item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(&ctl[kCtlQueue], 1u);
-        item = __builtin_amdgcn_readlane(item, 0);
+        const uint32_t item = next_item;
         if (item >= nwtiles) break;
         const TileRef tr = wtiles[item];
         const SimDesc& S = sim_at(sims, tr.sim);
@@ -1827,6 +1851,11 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
                 if (ev_hash) ev_hash[body] = ev.h;
             }
         }
+        // the next item is claimed when this one is done (a claim made earlier
+        // holds the item back from wavefronts that finish first)
+        uint32_t claimed = 0;
+        if (lane == 0) claimed = atomicAdd(&ctl[kCtlQueue], 1u);
+        next_item = grid_waves + __builtin_amdgcn_readlane(claimed, 0);
     }
 }
 

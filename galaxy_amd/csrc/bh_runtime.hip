@@ -267,20 +267,17 @@ struct SimSet {
         HIP_OK(hipGetLastError());
     }
     void sort(hipStream_t s) {
-        // the last pass (which also writes the sorted body copy) is its own
-        // instance, so its registers do not limit the occupancy of the others
+        const dim3 grid((uint32_t)tsort.size()), block(kSortThreads);
+        const uint32_t nt = (uint32_t)tsort.size();
         for (uint32_t pass = 0; pass < 4; ++pass) {
-            const dim3 grid((uint32_t)tsort.size()), block(kSortThreads);
-            const uint32_t nt = (uint32_t)tsort.size();
-            if (sort_kpt == 8) {
-                if (pass < 3) hipLaunchKernelGGL((k_sort_pass<8, false>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
-                else hipLaunchKernelGGL((k_sort_pass<8, true>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
-            } else {
-                if (pass < 3) hipLaunchKernelGGL((k_sort_pass<4, false>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
-                else hipLaunchKernelGGL((k_sort_pass<4, true>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
-            }
+            if (sort_kpt == 8)
+                hipLaunchKernelGGL((k_sort_pass<8>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
+            else
+                hipLaunchKernelGGL((k_sort_pass<4>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
             HIP_OK(hipGetLastError());
         }
+        hipLaunchKernelGGL(k_gather_sorted, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
+        HIP_OK(hipGetLastError());
     }
     void topo(hipStream_t s) {
         hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
