@@ -46,8 +46,12 @@ enum : uint32_t {
     kErrSpanCapacity = 4u,
 };
 
-// Traversal record, 32 bytes, in flat.rs preorder.  Non-leaf cells descend to
-// index + 1; every cell skips to `skip` (the next cell after its subtree).
+// Cell record, 32 bytes, in flat.rs preorder: f32 aggregates, the opening
+// threshold, the sorted range [start, end) and info (depth, leaf flag, span
+// slot).  The evidence readback and the aggregation read it; the traversal
+// reads the node stream instead.  `skip` (the next cell after the subtree) is
+// filled only for host-built trees, where the packing uses it to lay out their
+// node stream; device builds leave it 0.
 struct __attribute__((aligned(32))) CellT {
     float comx, comy, mass, T;  // T: smallest d2 that passes the opening test
     uint32_t start, end, skip, info;
