@@ -47,6 +47,9 @@ int guarded(F&& f) {
     } catch (const std::bad_alloc&) {
         set_error("host allocation failed");
         return -1;
+    } catch (const std::exception& e) {  // no C++ exception crosses the C ABI
+        set_error(std::string("internal error: ") + e.what());
+        return -1;
     }
 }
 
@@ -778,14 +781,16 @@ int gbh_final_kick(gbh_state* st, double dt, gbh_stage_timing* timing) {
 
 // ---------------------------------------------------------------------------- runtime/tree
 int gbh_runtime_create(gbh_device* dev, gbh_runtime** out) {
-    if (!dev || !out) {
-        set_error("null argument");
-        return -1;
-    }
-    auto* rt = new gbh_runtime();
-    rt->dev = dev;
-    *out = rt;
-    return 0;
+    return guarded([&] {
+        if (!dev || !out) {
+            set_error("null argument");
+            return -1;
+        }
+        auto* rt = new gbh_runtime();
+        rt->dev = dev;
+        *out = rt;
+        return 0;
+    });
 }
 
 void gbh_runtime_destroy(gbh_runtime* rt) { delete rt; }
