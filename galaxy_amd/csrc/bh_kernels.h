@@ -30,12 +30,11 @@ constexpr uint32_t kInfoLeaf = 1u << 8;
 constexpr uint32_t kInfoSpanShift = 9;   // (span slot + 1) << 9 for internal cells that cross tiles
 constexpr uint32_t kSpanSlotsLds = 4096; // spanning cells aggregated in LDS
 
-// Control words shared by the kernels of one simulation set.
+// Control words shared by the kernels of one simulation set (tile tickets are
+// per simulation, SimMeta::ticket).
 enum : uint32_t {
-    kCtlSortTicket = 0,  // unused (tile tickets are per simulation, SimMeta::ticket)
-    kCtlTopoTicket = 4,  // unused
-    kCtlQueue = 5,
-    kCtlError = 6,
+    kCtlQueue = 5,  // traversal work queue
+    kCtlError = 6,  // sticky error bits
     kCtlWords = 16,
 };
 
@@ -508,7 +507,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t i = tr.base + threadIdx.x;
     const bool valid = i < S.n;
-    if (blockIdx.x == 0 && threadIdx.x < 6) ctl[threadIdx.x] = 0u;  // tickets + queue
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlQueue] = 0u;
     // one look-back flag of the topology scan per block (there are fewer
     // topology tiles than bounds tiles)
     if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0ull;

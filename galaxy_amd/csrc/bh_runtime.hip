@@ -156,6 +156,8 @@ struct SimSet {
     uint32_t* d_tsum = nullptr;  // per topology tile boundary summaries (k_topo_pre)
     uint32_t* d_psum = nullptr;  // and their scans over the simulation's tiles (k_topo_scan)
     uint32_t* d_errs = nullptr;  // collected error words (k_collect_errors)
+    uint32_t* h_errs = nullptr;  // their pinned host copy
+    size_t cap_herrs = 0;
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
     size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0, cap_errs = 0;
     uint32_t sort_kpt = 8;
@@ -176,6 +178,9 @@ struct SimSet {
         d_tflags = nullptr;
         cap_desc = cap_t256 = cap_tsort = cap_ttopo = cap_twave = cap_sflags = cap_tflags = 0;
         cap_tsum = cap_psum = cap_errs = 0;
+        if (h_errs) (void)hipHostFree(h_errs);
+        h_errs = nullptr;
+        cap_herrs = 0;
     }
 
     template <class T>
@@ -337,22 +342,37 @@ struct SimSet {
 
     // Sticky error words of the set and of each simulation.
     // One gather kernel and one copy, whatever the number of simulations.
-    uint32_t read_errors(hipStream_t s, std::vector<uint32_t>* per_sim = nullptr) {
+    // enqueue_errors puts the gather and its copy into pinned host memory on the
+    // stream; take_errors reads the words once the stream has been synchronized.
+    void enqueue_errors(hipStream_t s) {
         const uint32_t nsims = (uint32_t)desc.size();
         ensure(d_errs, cap_errs, 1ull + nsims);
+        if (cap_herrs < 1ull + nsims) {
+            if (h_errs) HIP_OK(hipHostFree(h_errs));
+            h_errs = nullptr;
+            cap_herrs = 0;
+            HIP_OK(hipHostMalloc((void**)&h_errs, 4ull * (1ull + nsims), hipHostMallocDefault));
+            cap_herrs = 1ull + nsims;
+        }
         hipLaunchKernelGGL(k_collect_errors, dim3(div_up(nsims + 1u, 256u)), dim3(256), 0, s, d_desc, nsims,
                            d_ctl, d_errs);
         HIP_OK(hipGetLastError());
-        std::vector<uint32_t> words(1ull + nsims, 0);
-        HIP_OK(hipMemcpyAsync(words.data(), d_errs, 4ull * words.size(), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipMemcpyAsync(h_errs, d_errs, 4ull * (1ull + nsims), hipMemcpyDeviceToHost, s));
+    }
+    uint32_t take_errors(std::vector<uint32_t>* per_sim = nullptr) const {
+        const uint32_t nsims = (uint32_t)desc.size();
         uint32_t all = 0;
-        for (uint32_t e : words) all |= e;
-        if (per_sim) {  // a set-wide error (words[0]) taints every simulation of the set
+        for (uint32_t i = 0; i <= nsims; ++i) all |= h_errs[i];
+        if (per_sim) {  // a set-wide error (word 0) taints every simulation of the set
             per_sim->resize(nsims);
-            for (uint32_t i = 0; i < nsims; ++i) (*per_sim)[i] = words[1 + i] | words[0];
+            for (uint32_t i = 0; i < nsims; ++i) (*per_sim)[i] = h_errs[1 + i] | h_errs[0];
         }
         return all;
+    }
+    uint32_t read_errors(hipStream_t s, std::vector<uint32_t>* per_sim = nullptr) {
+        enqueue_errors(s);
+        HIP_OK(hipStreamSynchronize(s));
+        return take_errors(per_sim);
     }
 };
 
@@ -561,6 +581,9 @@ struct gbh_tree {
     char* arena = nullptr;
     size_t bytes = 0;
     bool built = false;
+    // The traversal queue counter is zero: a rebuild's k_bounds cleared it and
+    // no traversal ran since, so the next force needs no reset kernel.
+    bool queue_zero = false;
     SimSet set;
     hipEvent_t ev[8] = {};
     double last_force_ms = 0.0;
@@ -828,7 +851,9 @@ static int tree_create(gbh_runtime* rt, gbh_state* st, gbh_tree** out, uint32_t 
         t->set.build_tables();
         t->set.upload(rt->dev->stream);
         HIP_OK(hipMemsetAsync(tmp.meta, 0, sizeof(SimMeta), rt->dev->stream));
-        for (auto& e : t->ev) HIP_OK(hipEventCreate(&e));
+        // timing-only events (read after a stream synchronize): no system-scope
+        // fence at each record, which would stall the stream between stages
+        for (auto& e : t->ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         HIP_OK(hipStreamSynchronize(rt->dev->stream));
         *out = t.release();
         return 0;
@@ -877,6 +902,7 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
         double t0 = now_seconds();
         HIP_OK(hipEventRecord(t->ev[0], s));
         t->set.bounds(0, s);
+        t->queue_zero = true;
         HIP_OK(hipEventRecord(t->ev[1], s));
         t->set.keys(s);
         HIP_OK(hipEventRecord(t->ev[2], s));
@@ -886,9 +912,10 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
         HIP_OK(hipEventRecord(t->ev[4], s));
         t->set.agg(s);
         HIP_OK(hipEventRecord(t->ev[5], s));
+        t->set.enqueue_errors(s);
         HIP_OK(hipStreamSynchronize(s));
         double wall = now_seconds() - t0;
-        uint32_t err = t->set.read_errors(s);
+        uint32_t err = t->set.take_errors();
         if (err) {
             set_error("BH #2D rebuild failed on the device: " + error_text(err));
             return -1;
@@ -951,7 +978,8 @@ static int force_impl(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_con
         if (hash) HIP_OK(hipMalloc((void**)&d_hash, 8ull * t->n));
         if (d_stats || d_hash) t->set.fill_body_cold(s);
         HIP_OK(hipEventRecord(t->ev[6], s));
-        t->set.force(s, true, d_stats, d_hash);
+        t->set.force(s, !t->queue_zero, d_stats, d_hash);
+        t->queue_zero = false;
         HIP_OK(hipEventRecord(t->ev[7], s));
         HIP_OK(hipStreamSynchronize(s));
         double t2 = now_seconds();
