@@ -173,6 +173,14 @@ __device__ __forceinline__ const SimDesc& sim_at(const SimDesc* sims, uint32_t i
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Workgroup barrier that orders LDS accesses only (no wait for outstanding
+// global stores): for phases that exchange data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t dpp_shr_u32(uint32_t v, uint32_t identity, int ctrl_sel) {
     switch (ctrl_sel) {
     case 1: return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, 0x111, 0xf, 0xf, false);
@@ -1511,6 +1519,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     // same memory holds the spanning cells bucketed by depth (kTopListCap slots).
     __shared__ double s_mxy[3 * kTopLds];
     __shared__ uint32_t s_ref[kTopLds / 5][4];
+    __shared__ uint32_t s_idx[kTopLds / 5], s_node[kTopLds / 5];  // cell and node index
     __shared__ uint8_t s_depth[kTopLds / 5];
     __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
     double* s_m = s_mxy;
@@ -1529,7 +1538,11 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     if (in_lds) {
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
             const uint32_t idx = S.span[i];
-            s_depth[i] = (uint8_t)(S.cells[idx].info & 255u);
+            const uint32_t depth = S.cells[idx].info & 255u;
+            const uint32_t st = S.cells[idx].start;
+            s_idx[i] = idx;
+            s_depth[i] = (uint8_t)depth;
+            s_node[i] = S.nbase[st] + (depth - S.cdmin[st]);
             const uint4 ch = S.cchild[idx];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1573,18 +1586,38 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-            const uint32_t d = min(S.cells[S.span[i]].info & 255u, kLevels - 1u);
-            s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = i;
+            const uint32_t idx = S.span[i];
+            const uint32_t d = min(S.cells[idx].info & 255u, kLevels - 1u);
+            s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = idx;
         }
         __syncthreads();
     }
     for (int d = 15; d >= 0; --d) {
         const uint32_t lo = listed ? s_off[d] : 0u, hi = listed ? s_off[d + 1] : nspan;
         for (uint32_t k = lo + threadIdx.x; k < hi; k += kTopThreads) {
-            const uint32_t i = listed ? s_list[k] : k;
-            const uint32_t idx = S.span[i];
-            const uint32_t depth = in_lds ? s_depth[i] : (S.cells[idx].info & 255u);
+            // the cell (index i of the span list, cell idx) and its depth
+            const uint32_t i = k;
+            uint32_t idx, depth;
+            if (in_lds) {
+                depth = s_depth[i];
+                idx = s_idx[i];
+            } else if (listed) {  // bucket d holds the cells of depth d
+                idx = s_list[k];
+                depth = (uint32_t)d;
+            } else {
+                idx = S.span[i];
+                depth = S.cells[idx].info & 255u;
+            }
             if (depth != (uint32_t)d) continue;
+            // node index first: its loads issued after this cell's stores would
+            // wait for them
+            uint32_t node;
+            if (in_lds) {
+                node = s_node[i];
+            } else {
+                const uint32_t st = S.cells[idx].start;
+                node = S.nbase[st] + (depth - S.cdmin[st]);
+            }
             double mass = 0.0, wx = 0.0, wy = 0.0;
             if (in_lds) {
 #pragma unroll
@@ -1618,13 +1651,15 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             S.cells[idx].comx = (float)comx;
             S.cells[idx].comy = (float)comy;
             S.cells[idx].mass = (float)mass;
-            const uint32_t st = S.cells[idx].start;
-            NodeHot* nd = &S.nodes[S.nbase[st] + (depth - S.cdmin[st])];
+            NodeHot* nd = &S.nodes[node];
             nd->x = (float)comx;
             nd->y = (float)comy;
             nd->m = (float)mass;
         }
-        __syncthreads();
+        // In LDS mode the next depth reads only LDS, so the barrier need not
+        // wait for this depth's global stores; otherwise it reads them.
+        if (in_lds) lds_barrier();
+        else __syncthreads();
     }
 }
 
