@@ -173,6 +173,16 @@ __device__ __forceinline__ const SimDesc& sim_at(const SimDesc* sims, uint32_t i
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Tile of workgroup b among n when the hardware deals workgroups to the 8 XCDs
+// round-robin (b to XCD b % 8): each XCD gets one contiguous run of tiles, so
+// the simulations whose data its L2 holds are few.  A bijection on [0, n)
+// whatever the dealing is.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // Workgroup barrier that orders LDS accesses only (no wait for outstanding
 // global stores): for phases that exchange data through LDS alone.
 __device__ __forceinline__ void lds_barrier() {
@@ -603,7 +613,8 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
     __shared__ float s_red[4][kSortThreads / 64];
     __shared__ double s_root[3];
     __shared__ uint32_t s_hist[4][kRadix];
-    const TileRef tr = stiles[blockIdx.x];
+    const uint32_t tb = xcd_tile(blockIdx.x, gridDim.x);
+    const TileRef tr = stiles[tb];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     // Every load of the kernel is issued before its first store: the positions
@@ -627,7 +638,7 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
     // reset this tile's look-back flags for the four radix passes
     for (uint32_t k = threadIdx.x; k < 4 * kRadix; k += kSortThreads) {
         uint32_t pass = k / kRadix, d = k % kRadix;
-        sflags[((size_t)pass * nstiles + blockIdx.x) * kRadix + d] = 0u;
+        sflags[((size_t)pass * nstiles + tb) * kRadix + d] = 0u;
         s_hist[pass][d] = 0u;
     }
 #pragma unroll
@@ -741,7 +752,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     // in order and a look-back only waits for tiles whose workgroups run.
     // (One ticket counter per simulation: a single counter for every tile
     // serialises on one address.)
-    const uint32_t sim = stiles[blockIdx.x].sim;
+    const uint32_t sim = stiles[xcd_tile(blockIdx.x, gridDim.x)].sim;
     const SimDesc& S = sim_at(sims, sim);
     if (threadIdx.x == 0) s_ticket = S.first_sort_tile + atomicAdd(&S.meta->ticket[pass], 1u);
     for (uint32_t k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&s_wcnt[0][0])[k] = 0u;
@@ -847,7 +858,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
 // workgroup (and the look-back of the tiles after it).
 __global__ __launch_bounds__(kTile) void k_gather_sorted(const SimDesc* __restrict__ sims,
                                                          const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[blockIdx.x];
+    const TileRef tr = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t p = tr.base + threadIdx.x;
     if (p >= S.n) return;
@@ -894,7 +905,7 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __rest
                                                            uint32_t* __restrict__ tsum) {
     // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
     __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
@@ -1092,7 +1103,7 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
     // simulation from the block index, tile from a ticket of that simulation
     // (as in k_sort_pass)
-    const uint32_t sim = ttiles[blockIdx.x].sim;
+    const uint32_t sim = ttiles[blockIdx.x].sim;  // (an XCD-contiguous order measured slower here)
     const SimDesc& S = sim_at(sims, sim);
     if (threadIdx.x == 0) s_ticket = S.first_topo_tile + atomicAdd(&S.meta->ticket[4], 1u);
     __syncthreads();
@@ -1348,7 +1359,7 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
     __shared__ uint32_t s_ch[kLinkCells][4];
     __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];
     __shared__ uint32_t s_cb0, s_cb1, s_dlo, s_dhi;
-    const TileRef tr = ttiles[blockIdx.x];
+    const TileRef tr = ttiles[xcd_tile(blockIdx.x, gridDim.x)];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     const uint32_t p = tr.base + threadIdx.x;
