@@ -525,6 +525,14 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t i = tr.base + threadIdx.x;
     const bool valid = i < S.n;
+    // the body's loads first (a load issued after a store waits for it)
+    const uint32_t ic = min(i, S.n - 1u);
+    float2 p = S.pos[ic];
+    float2 v = make_float2(0.f, 0.f), a = make_float2(0.f, 0.f);
+    if (mode != 0) {
+        v = S.vel[ic];
+        a = S.acc[ic];
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlQueue] = 0u;
     // one look-back flag of the topology scan per block (there are fewer
     // topology tiles than bounds tiles)
@@ -535,12 +543,9 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     }
     float x = 0.f, y = 0.f;
     if (valid) {
-        float2 p = S.pos[i];
         if (mode != 0) {
             const float dt = S.dt;
             const float h = 0.5f * dt;
-            float2 v = S.vel[i];
-            float2 a = S.acc[i];
             if (mode == 2) {  // bh_final_kick of the previous step
                 v.x = v.x + h * a.x;
                 v.y = v.y + h * a.y;
@@ -571,7 +576,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
         s_red[2][w] = mxx;
         s_red[3][w] = mxy;
     }
-    __syncthreads();
+    lds_barrier();  // the reduction reads LDS only: no wait for the state stores
     if (threadIdx.x == 0) {
         for (uint32_t k = 1; k < kTile / 64; ++k) {
             mnx = fminf(mnx, s_red[0][k]);
