@@ -450,22 +450,43 @@ bool validate_states(const GbhEvolveState* s, uint32_t n) {
 }
 
 // Upload frozen records into SoA state through a staging buffer.
-void upload_state(const SimDesc& d, const GbhEvolveState* host, uint32_t n, hipStream_t s) {
-    GbhEvolveState* staging = nullptr;
-    HIP_OK(hipMalloc((void**)&staging, sizeof(GbhEvolveState) * n));
+// Device staging buffer for the record <-> SoA state transfers, grown on demand
+// and kept by its owner (hipMalloc is slow and hipFree synchronizes the device,
+// so a server does neither per request).
+struct Staging {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* get(size_t bytes) {
+        if (bytes > cap) {
+            if (p) HIP_OK(hipFree(p));
+            p = nullptr;
+            cap = 0;
+            HIP_OK(hipMalloc(&p, bytes));
+            cap = bytes;
+        }
+        return p;
+    }
+    ~Staging() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+void upload_state(const SimDesc& d, const GbhEvolveState* host, uint32_t n, hipStream_t s,
+                  Staging* keep = nullptr) {
+    Staging local;
+    GbhEvolveState* staging = (GbhEvolveState*)(keep ? keep : &local)->get(sizeof(GbhEvolveState) * n);
     HIP_OK(hipMemcpyAsync(staging, host, sizeof(GbhEvolveState) * n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_unpack_state, dim3(div_up(n, 256)), dim3(256), 0, s, staging, n, d.pos,
                        d.mass, d.vel, d.acc);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s));
-    HIP_OK(hipFree(staging));
 }
 
 void download_state(const SimDesc& d, GbhEvolveState* out, GbhGpuAccel* acc, uint32_t n,
-                    hipStream_t s) {
+                    hipStream_t s, Staging* keep = nullptr) {
     size_t bytes = std::max(sizeof(GbhEvolveState), sizeof(GbhGpuAccel)) * (size_t)n;
-    void* staging = nullptr;
-    HIP_OK(hipMalloc(&staging, bytes));
+    Staging local;
+    void* staging = (keep ? keep : &local)->get(bytes);
     if (out) {
         hipLaunchKernelGGL(k_pack_state, dim3(div_up(n, 256)), dim3(256), 0, s,
                            (GbhEvolveState*)staging, n, d.pos, d.mass, d.vel);
@@ -480,7 +501,6 @@ void download_state(const SimDesc& d, GbhEvolveState* out, GbhGpuAccel* acc, uin
         HIP_OK(hipMemcpyAsync(acc, staging, sizeof(GbhGpuAccel) * n, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
     }
-    HIP_OK(hipFree(staging));
 }
 
 // Host copy of a built tree: everything evidence readbacks need.
