@@ -107,6 +107,23 @@ def test_server_budget_admission_and_staggered_arrivals(oracle, gpu):
         Server(gpu, hbm_budget_bytes=1 << 20, steps_per_round=10).submit(*sims[1], 10)
 
 
+def test_server_reuses_retired_arenas(oracle, gpu):
+    """The budget admits one request at a time, so each later request runs in the
+    arena a retired one left behind (same size, then a smaller size within 2x), with
+    that request's data still in it; results still equal the oracle's."""
+    cfg = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    srv = Server(gpu, hbm_budget_bytes=Server.request_bytes(6000) + 1, steps_per_round=10)
+    for n, seed, k in [(6000, 11, 10), (6000, 12, 20), (4000, 13, 10)]:
+        s0 = ics.rotating_disc(n, seed)
+        print(f"server arena reuse: n={n} seed={seed} steps={k}", flush=True)
+        rid = srv.submit(s0, cfg, 1e-3, k)
+        srv.drain()
+        rs, ra = oracle.f32_evolve(s0, k, 1e-3, ocfg(oracle, cfg))
+        s, a = srv.take(rid)
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), n
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), n
+
+
 def test_tuning_rejects_unsupported_shapes(gpu):
     sims = mixed_sims()[:1]
     for kw, msg in [(dict(sort_tile=512), "sort_tile"), (dict(traverse_block=96), "traverse_block"),
