@@ -31,11 +31,17 @@ constexpr uint32_t kInfoSpanShift = 9;   // (span slot + 1) << 9 for internal ce
 constexpr uint32_t kSpanSlotsLds = 4096; // spanning cells aggregated in LDS
 
 // Control words shared by the kernels of one simulation set (tile tickets are
-// per simulation, SimMeta::ticket).
+// per simulation, SimMeta::ticket): the sticky error bits, the traversal's work
+// distribution (mode, the item range of each XCD: start[8], length[8]; set by
+// the host) and one work-queue counter per XCD, each in its own 128-byte line.
+constexpr uint32_t kXcds = 8;
 enum : uint32_t {
-    kCtlQueue = 5,  // traversal work queue
-    kCtlError = 6,  // sticky error bits
-    kCtlWords = 16,
+    kCtlError = 6,
+    kCtlMode = 7,  // 0: one item range per XCD; 1: one range for the whole grid
+    kCtlRanges = 16,
+    kCtlQueue0 = 32,
+    kQueueStride = 32,
+    kCtlWords = kCtlQueue0 + kXcds * kQueueStride,
 };
 
 // Error bits (sticky in kCtlError and SimMeta::err).
@@ -177,7 +183,6 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // round-robin (b to XCD b % 8): each XCD gets one contiguous run of tiles, so
 // the simulations whose data its L2 holds are few.  A bijection on [0, n)
 // whatever the dealing is.
-constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
     const uint32_t q = n / kXcds, r = n % kXcds, x = b % kXcds;
     return x * q + min(x, r) + b / kXcds;
@@ -533,7 +538,7 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
         v = S.vel[ic];
         a = S.acc[ic];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlQueue] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) ctl[kCtlQueue0 + threadIdx.x * kQueueStride] = 0u;
     // one look-back flag of the topology scan per block (there are fewer
     // topology tiles than bounds tiles)
     if (threadIdx.x == 0 && blockIdx.x < ntopo) tflags[blockIdx.x] = 0ull;
@@ -588,8 +593,8 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     }
 }
 
-__global__ void k_reset_ctl(uint32_t* __restrict__ ctl, uint32_t first, uint32_t count) {
-    if (threadIdx.x < count) ctl[first + threadIdx.x] = 0u;
+__global__ void k_reset_queues(uint32_t* __restrict__ ctl) {
+    if (threadIdx.x < kXcds) ctl[kCtlQueue0 + threadIdx.x * kQueueStride] = 0u;
 }
 
 // Sticky error words of a set into one array: out[0] = the set's control error
@@ -1829,6 +1834,50 @@ struct NodeWalk {
     }
 };
 
+// Traversal work items by XCD (mode 0).  For a set of many simulations the host
+// lays the items out as one contiguous, balanced range per XCD (simulations
+// dealt to the XCDs in size order), so an XCD's L2 holds the trees of few
+// simulations.  A wavefront of workgroup b runs on XCD b % 8 (the hardware deals
+// workgroups round-robin; were it otherwise, only locality would suffer): its
+// first item is its index among that XCD's wavefronts, later ones come from that
+// XCD's queue counter, and when the range is exhausted it takes items from
+// another XCD's queue.  A set whose largest simulation would have to be split
+// across XCDs uses mode 1 instead: one range, first items by grid index, one
+// counter (measured faster there: the whole grid walks one window of targets).
+__device__ __forceinline__ uint32_t xcd_waves(uint32_t x, uint32_t wpb) {
+    return ((gridDim.x + kXcds - 1u - x) / kXcds) * wpb;  // wavefronts on XCD x
+}
+
+// Next item from queue q (or, once q is exhausted, from any queue that still
+// holds items); nitems when none does.  Every failed claim leaves its queue
+// exhausted for good (counters only grow), so the loop ends within kXcds + 1
+// claims.  Mode 1: the grid's one counter.  Called by whole wavefronts.
+__device__ uint32_t claim_item(uint32_t* ctl, uint32_t mode, uint32_t& q, uint32_t wpb, uint32_t nitems) {
+    const uint32_t lane = lane_id();
+    if (mode != 0u) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&ctl[kCtlQueue0], 1u);
+        k = __builtin_amdgcn_readlane(k, 0);
+        const unsigned long long item = (unsigned long long)gridDim.x * wpb + k;
+        return item < nitems ? (uint32_t)item : nitems;
+    }
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&ctl[kCtlQueue0 + q * kQueueStride], 1u);
+        k = __builtin_amdgcn_readlane(k, 0);
+        const unsigned long long j = (unsigned long long)xcd_waves(q, wpb) + k;
+        if (j < ctl[kCtlRanges + kXcds + q]) return ctl[kCtlRanges + q] + (uint32_t)j;
+        bool open = false;
+        if (lane < kXcds) {
+            const uint32_t c = flag_load(&ctl[kCtlQueue0 + lane * kQueueStride]);
+            open = (unsigned long long)xcd_waves(lane, wpb) + c < ctl[kCtlRanges + kXcds + lane];
+        }
+        const unsigned long long m = __ballot(open);
+        if (m == 0ull) return nitems;
+        q = (uint32_t)__builtin_ctzll(m);
+    }
+}
+
 template <bool kEvidence>
 __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restrict__ sims,
                                                         const TileRef* __restrict__ wtiles,
@@ -1841,11 +1890,19 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
     const uint32_t tab = __builtin_amdgcn_readfirstlane(
         (uint32_t)(uintptr_t)(__attribute__((address_space(3))) float*)tabp);
     const uint32_t lane = lane_id();
-    // The first item of every wavefront is its index in the grid; later items
-    // come from the queue counter, offset by the grid's wavefront count (so
-    // the launch does not start with every wavefront on one atomic).
-    const uint32_t grid_waves = gridDim.x * (blockDim.x >> 6);
-    uint32_t next_item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // first item: the wavefront's index among its XCD's wavefronts (so the
+    // launch does not start with every wavefront on an atomic)
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t mode = __builtin_amdgcn_readfirstlane(ctl[kCtlMode]);
+    uint32_t queue = blockIdx.x % kXcds;
+    uint32_t next_item;
+    if (mode != 0u) {
+        next_item = blockIdx.x * wpb + (threadIdx.x >> 6);
+    } else {
+        const uint32_t j0 = (blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6);
+        next_item = j0 < ctl[kCtlRanges + kXcds + queue] ? ctl[kCtlRanges + queue] + j0
+                                                          : claim_item(ctl, mode, queue, wpb, nwtiles);
+    }
     for (;;) {
         const uint32_t item = next_item;
         if (item >= nwtiles) break;
@@ -1907,9 +1964,7 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         }
         // the next item is claimed when this one is done (a claim made earlier
         // holds the item back from wavefronts that finish first)
-        uint32_t claimed = 0;
-        if (lane == 0) claimed = atomicAdd(&ctl[kCtlQueue], 1u);
-        next_item = grid_waves + __builtin_amdgcn_readlane(claimed, 0);
+        next_item = claim_item(ctl, mode, queue, wpb, nwtiles);
     }
 }
 

@@ -160,6 +160,8 @@ struct SimSet {
     size_t cap_herrs = 0;
     size_t cap_desc = 0, cap_t256 = 0, cap_tsort = 0, cap_ttopo = 0, cap_twave = 0;
     size_t cap_sflags = 0, cap_tflags = 0, cap_tsum = 0, cap_psum = 0, cap_errs = 0;
+    uint32_t mode = 0;                // traversal work distribution (kCtlMode)
+    uint32_t ranges[2 * kXcds] = {};  // traversal items per XCD: start[8], length[8]
     uint32_t sort_kpt = 8;
     uint32_t trav_block = 256;        // traversal workgroup size (64, 128 or 256)
     uint32_t trav_waves_per_cu = 32;  // persistent traversal wavefronts per CU
@@ -211,13 +213,37 @@ struct SimSet {
             d.first_topo_tile = (uint32_t)ttopo.size();
             for (uint32_t b = 0; b < d.n; b += kTopoTile) ttopo.push_back({s, b});
         }
-        // traversal work items: larger simulations first (longer walks) for a short tail
+        // Traversal work items.  Mode 0 (many simulations): one contiguous range
+        // per XCD; simulations are dealt to the XCDs round-robin in size order
+        // (similar work per XCD, each simulation's items together) and the
+        // sequence of the XCDs' lists is cut into kXcds equal ranges.  Mode 1
+        // (a simulation larger than one XCD's share): one range in size order.
+        // Within a range, larger simulations come first (longer walks) for a
+        // short tail.
         std::vector<uint32_t> order(desc.size());
         for (uint32_t s = 0; s < order.size(); ++s) order[s] = s;
         std::stable_sort(order.begin(), order.end(),
                          [&](uint32_t a, uint32_t b) { return desc[a].n > desc[b].n; });
-        for (uint32_t s : order)
-            for (uint32_t b = 0; b < desc[s].n; b += kWaveTargets) twave.push_back({s, b});
+        uint64_t total = 0, largest = 0;
+        for (const SimDesc& d : desc) {
+            const uint64_t items = div_up(d.n, kWaveTargets);
+            total += items;
+            largest = std::max(largest, items);
+        }
+        mode = largest * kXcds > total ? 1u : 0u;
+        if (mode == 0u) {
+            for (uint32_t x = 0; x < kXcds; ++x)
+                for (size_t k = x; k < order.size(); k += kXcds)
+                    for (uint32_t b = 0; b < desc[order[k]].n; b += kWaveTargets) twave.push_back({order[k], b});
+        } else {
+            for (uint32_t s : order)
+                for (uint32_t b = 0; b < desc[s].n; b += kWaveTargets) twave.push_back({s, b});
+        }
+        const uint32_t nt = (uint32_t)twave.size(), q = nt / kXcds, r = nt % kXcds;
+        for (uint32_t x = 0; x < kXcds; ++x) {
+            ranges[x] = x * q + std::min(x, r);
+            ranges[kXcds + x] = q + (x < r ? 1u : 0u);
+        }
     }
 
     void upload(hipStream_t stream) {
@@ -244,6 +270,8 @@ struct SimSet {
         put(d_ttopo, ttopo);
         put(d_twave, twave);
         HIP_OK(hipMemsetAsync(d_ctl, 0, sizeof(uint32_t) * kCtlWords, stream));
+        HIP_OK(hipMemcpyAsync(d_ctl + kCtlMode, &mode, sizeof(mode), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(d_ctl + kCtlRanges, ranges, sizeof(ranges), hipMemcpyHostToDevice, stream));
         HIP_OK(hipMemsetAsync(d_sflags, 0, sizeof(uint32_t) * 4ull * kRadix * std::max<size_t>(tsort.size(), 1), stream));
         HIP_OK(hipMemsetAsync(d_tflags, 0, sizeof(unsigned long long) * std::max<size_t>(ttopo.size(), 1), stream));
         // persistent traversal grid: trav_waves_per_cu wavefronts per CU, no more
@@ -314,7 +342,7 @@ struct SimSet {
     void force(hipStream_t s, bool reset_queue, uint32_t* ev_stats = nullptr,
                unsigned long long* ev_hash = nullptr, uint32_t ev_sim = 0) {
         if (reset_queue) {
-            hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, s, d_ctl, (uint32_t)kCtlQueue, 1u);
+            hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, s, d_ctl);
             HIP_OK(hipGetLastError());
         }
         if (ev_stats || ev_hash)
