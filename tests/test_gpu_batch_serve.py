@@ -65,6 +65,34 @@ def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile, block, waves, buck
     b.close()
 
 
+@pytest.mark.parametrize("use_graph,block,waves", [(True, 256, 20), (False, 64, 16)])
+def test_batch_many_sims_per_xcd_work(oracle, gpu, use_graph, block, waves):
+    """Twenty simulations of uneven size: no simulation exceeds an eighth of the
+    traversal items, so the items are distributed by XCD (one range and queue per
+    XCD, stealing at the end).  Every simulation follows the oracle's trajectory,
+    and the evidence traversal (which runs the same distribution) matches too."""
+    disc = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    sizes = [3000, 40, 2100, 700, 1500, 129, 900, 2600, 64, 1800, 333, 1200, 2900, 517, 80, 2400, 1000,
+             260, 1700, 600]
+    assert max((n + 63) // 64 for n in sizes) * 8 <= sum((n + 63) // 64 for n in sizes)
+    states = [ics.rotating_disc(n, 5000 + n, scale=0.6 + (n % 7) * 0.2) for n in sizes]
+    print(f"batch of {len(sizes)} sims graph={use_graph} block={block} waves={waves}", flush=True)
+    b = Batch(gpu, states, disc, 1e-3, use_graph=use_graph, traverse_block=block, traverse_waves_per_cu=waves)
+    b.step(2)
+    b.step(1)
+    for i, s0 in enumerate(states):
+        s, a = b.read(i)
+        rs, ra = oracle.f32_evolve(s0, 3, 1e-3, ocfg(oracle, disc))
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+    k = sizes.index(2600)
+    st, hsh = b.force_evidence(k)
+    t = oracle.tree_from_state(b.read(k)[0], ocfg(oracle, disc))
+    _, ostats, ohash = oracle.traverse_f32(*t.pack(), evidence=True)
+    assert np.array_equal(st, ostats) and np.array_equal(hsh, ohash)
+    b.close()
+
+
 def test_server_results_same_alone_and_in_mix(oracle, gpu):
     sims = mixed_sims()
     steps = [10, 30, 20, 10, 40]
