@@ -862,22 +862,6 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     }
 }
 
-// Sorted body copy after the last radix pass: sbody[p] = (x, y, m) of the body
-// at sorted position p.  A kernel of its own: its reads go to random body
-// indices, and inside the last pass their latency held every tile's
-// workgroup (and the look-back of the tiles after it).
-__global__ __launch_bounds__(kTile) void k_gather_sorted(const SimDesc* __restrict__ sims,
-                                                         const TileRef* __restrict__ tiles) {
-    const TileRef tr = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const SimDesc& S = sim_at(sims, tr.sim);
-    const uint32_t p = tr.base + threadIdx.x;
-    if (p >= S.n) return;
-    const uint32_t v = S.val0[p];
-    const float2 q = S.pos[v];
-    const float m = S.mass[v];
-    S.sbody[p] = make_float4(q.x, q.y, m, 0.0f);
-}
-
 // ----------------------------------------------------------------------------
 // K4: cell columns.  Position p starts cells at depths dmin(p)..dleaf(p): dmin
 // from the common prefix with its predecessor, existence from the parent group's
@@ -909,7 +893,10 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 
 // Per topology tile: dmin of every position (cdmin) and, per depth d, the first
 // boundary position in the tile and one past the last (tsum[t][d], tsum[t][17+d];
-// kNone / 0 when the tile has none).
+// kNone / 0 when the tile has none).  Also the sorted body copy, sbody[p] =
+// (x, y, m) of the body at sorted position p: its reads go to random body
+// indices (inside the last radix pass their latency held every tile's
+// workgroup), here they overlap the boundary work.
 __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
                                                            const TileRef* __restrict__ ttiles,
                                                            uint32_t* __restrict__ tsum) {
@@ -921,10 +908,16 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __rest
     const uint32_t n = S.n;
     const uint32_t w = threadIdx.x >> 6;
     uint32_t dmin[kTopoPer];
+    float2 bq[kTopoPer];
+    float bm[kTopoPer];
+    // every load before the first store: keys, then the body gather
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+        const uint32_t v = S.val0[min(p, n - 1u)];
         dmin[j] = p < n ? dmin_of(S.key0, p, S.key0[p]) : 17u;
+        bq[j] = S.pos[v];
+        bm[j] = S.mass[v];
     }
     for (uint32_t d = 0; d < kLevels; ++d) {
         uint32_t first = kNone, last = 0u;
@@ -958,6 +951,11 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __rest
         }
         tsum[(size_t)t * kTopoSum + threadIdx.x] = first;
         tsum[(size_t)t * kTopoSum + kLevels + threadIdx.x] = last;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+        if (p < n) S.sbody[p] = make_float4(bq[j].x, bq[j].y, bm[j], 0.0f);
     }
 }
 

@@ -312,8 +312,6 @@ struct SimSet {
                 hipLaunchKernelGGL((k_sort_pass<4>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
             HIP_OK(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_gather_sorted, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
-        HIP_OK(hipGetLastError());
     }
     void topo(hipStream_t s) {
         hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
