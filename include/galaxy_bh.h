@@ -282,6 +282,9 @@ typedef struct {
 /* Device bytes one resident simulation of n bodies holds (state + tree buffers);
  * the unit of HBM-budget admission. */
 uint64_t gbh_sim_device_bytes(uint32_t n);
+/* The server keeps the device arenas of retired requests for reuse by later ones
+ * (up to hbm_budget_bytes of them, on top of the resident requests); they are
+ * released when an allocation fails and at gbh_server_destroy. */
 int gbh_server_create(gbh_device* dev, const gbh_server_options* opt, gbh_server** out);
 void gbh_server_destroy(gbh_server* s);
 /* Server clock in seconds (monotonic, starts at create). */
