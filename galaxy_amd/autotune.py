@@ -13,7 +13,8 @@ The search space:
 The launch-shape choices leave every result bit-identical; the leaf size defines
 the tree, and the CPU restatement reproduces the tree and forces for any leaf size.
 Candidates are timed on the workload itself (whole HIP-graph steps after a
-warm-up) by coordinate descent from the defaults, and the winner is cached per
+warm-up) by coordinate descent from the defaults (passes repeated while one
+improves, at most three), and the winner is cached per
 workload signature (sizes histogram, softening, theta, device) in a JSON file,
 so later runs of the same ensemble shape reuse it without measuring.
 """
@@ -104,11 +105,15 @@ def tune(gpu: NbodyGpu, states, config: Config, dt: float, cache: str | None = D
         return seen[c]
 
     cost(best)
-    for name, values in SPACE.items():
-        for v in values:
-            cand = replace(best, **{name: v})
-            if cost(cand) < cost(best):
-                best = cand
+    for _ in range(3):  # coordinate descent, repeated while a pass improves (timings are kept)
+        start = best
+        for name, values in SPACE.items():
+            for v in values:
+                cand = replace(best, **{name: v})
+                if cost(cand) < cost(best):
+                    best = cand
+        if best == start:
+            break
     result = {"choice": best, "seconds_per_step": seen[best],
               "measured": [(asdict(c), s) for c, s in seen.items()], "cached": False, "signature": key}
     if cache:
