@@ -143,7 +143,8 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
         log(f"  tuned: {asdict(choice)}")
     cfg = replace(w.config, bucket=choice.bucket)
     b = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=choice.sort_tile,
-              traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu)
+              traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu,
+              streams=choice.streams)
     total = w.bodies
     b.step(max(warmup, 1))
     t0 = time.perf_counter()

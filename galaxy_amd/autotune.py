@@ -9,6 +9,8 @@ The search space:
   sort_tile              keys per radix-sort workgroup: 2048, 1024
   traverse_block         traversal workgroup size: 256, 128, 64 threads
   traverse_waves_per_cu  persistent traversal wavefronts per CU: 32, 24, 20, 16
+  streams                groups of simulations stepped on streams of their own, so
+                         their kernels overlap: 1, 2
 
 The launch-shape choices leave every result bit-identical; the leaf size defines
 the tree, and the CPU restatement reproduces the tree and forces for any leaf size.
@@ -36,6 +38,7 @@ SPACE = {
     "sort_tile": (2048, 1024),
     "traverse_block": (256, 128, 64),
     "traverse_waves_per_cu": (32, 24, 20, 16),
+    "streams": (1, 2),
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "galaxy_amd", "tune.json")
@@ -47,6 +50,7 @@ class Choice:
     sort_tile: int = 2048
     traverse_block: int = 256
     traverse_waves_per_cu: int = 32
+    streams: int = 1
 
 
 def signature(sizes, config: Config, dt: float, device: str) -> str:
@@ -64,7 +68,8 @@ def measure(gpu: NbodyGpu, states, config: Config, dt: float, choice: Choice, st
     outside the timed steps."""
     cfg = replace(config, bucket=choice.bucket)
     b = Batch(gpu, states, cfg, dt, use_graph=True, sort_tile=choice.sort_tile,
-              traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu)
+              traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu,
+              streams=choice.streams)
     try:
         b.step(warmup)
         t0 = time.perf_counter()

@@ -65,19 +65,24 @@ def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile, block, waves, buck
     b.close()
 
 
-@pytest.mark.parametrize("use_graph,block,waves", [(True, 256, 20), (False, 64, 16)])
-def test_batch_many_sims_per_xcd_work(oracle, gpu, use_graph, block, waves):
+@pytest.mark.parametrize("use_graph,block,waves,streams", [(True, 256, 20, 1), (False, 64, 16, 1),
+                                                          (True, 256, 16, 2), (False, 128, 20, 2)])
+def test_batch_many_sims_per_xcd_work(oracle, gpu, use_graph, block, waves, streams):
     """Twenty simulations of uneven size: no simulation exceeds an eighth of the
     traversal items, so the items are distributed by XCD (one range and queue per
-    XCD, stealing at the end).  Every simulation follows the oracle's trajectory,
-    and the evidence traversal (which runs the same distribution) matches too."""
+    XCD, stealing at the end).  With streams=2, two groups of ten step concurrently
+    on two streams (each group holds a simulation larger than an eighth of its own
+    items, so it uses the grid-wide range).  Every simulation follows the oracle's
+    trajectory, and the evidence traversal matches too."""
     disc = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
     sizes = [3000, 40, 2100, 700, 1500, 129, 900, 2600, 64, 1800, 333, 1200, 2900, 517, 80, 2400, 1000,
              260, 1700, 600]
     assert max((n + 63) // 64 for n in sizes) * 8 <= sum((n + 63) // 64 for n in sizes)
     states = [ics.rotating_disc(n, 5000 + n, scale=0.6 + (n % 7) * 0.2) for n in sizes]
-    print(f"batch of {len(sizes)} sims graph={use_graph} block={block} waves={waves}", flush=True)
-    b = Batch(gpu, states, disc, 1e-3, use_graph=use_graph, traverse_block=block, traverse_waves_per_cu=waves)
+    print(f"batch of {len(sizes)} sims graph={use_graph} block={block} waves={waves} streams={streams}",
+          flush=True)
+    b = Batch(gpu, states, disc, 1e-3, use_graph=use_graph, traverse_block=block, traverse_waves_per_cu=waves,
+              streams=streams)
     b.step(2)
     b.step(1)
     for i, s0 in enumerate(states):
@@ -155,7 +160,7 @@ def test_server_reuses_retired_arenas(oracle, gpu):
 def test_tuning_rejects_unsupported_shapes(gpu):
     sims = mixed_sims()[:1]
     for kw, msg in [(dict(sort_tile=512), "sort_tile"), (dict(traverse_block=96), "traverse_block"),
-                    (dict(traverse_waves_per_cu=33), "traverse_waves_per_cu")]:
+                    (dict(traverse_waves_per_cu=33), "traverse_waves_per_cu"), (dict(streams=3), "streams")]:
         with pytest.raises(Exception, match=msg):
             Batch(gpu, [sims[0][0]], sims[0][1], sims[0][2], **kw)
 
