@@ -358,6 +358,9 @@ def main():
                               "WRITE_SIZE x%.3f per launch)" % (traffic["fetch_scale"], traffic["write_scale"])
                               if traffic else None,
             "interactions_per_launch": batched["interactions"],
+            # a step's traversal: one launch per group of simulations (config.batched_choice
+            # streams); kernel_ms sums the groups' launches, timed one after the other
+            "launches_per_step": int(batched["choice"].get("streams", 1)),
             "kernel_ms": trav_ms,
             "plain_traversal_tflops": plain_trav,
             "plain_traversal_ms": plain["traversal_ms"],
