@@ -138,7 +138,7 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
     choice = choice or autotune.Choice()
     if tune:
         log("  autotuning leaf size / sort tile / traversal launch shape ...")
-        tuning = autotune.tune(gpu, w.states, w.config, w.dt, cache=tune_cache, steps=5, log=log)
+        tuning = autotune.tune(gpu, w.states, w.config, w.dt, cache=tune_cache, steps=20, log=log)
         choice = tuning["choice"]
         log(f"  tuned: {asdict(choice)}")
     cfg = replace(w.config, bucket=choice.bucket)
