@@ -191,7 +191,8 @@ def serving_run(gpu, requests=300, rate_lo=50.0, rate_hi=2000.0, seed=777, choic
             s = ics.uniform_collapse(n, 10**6 + k)
             cfg, dt = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16), 5e-4
         reqs.append((at, s, cfg, dt, steps))
-    shape = {k: choice[k] for k in ("sort_tile", "traverse_block", "traverse_waves_per_cu")} if choice else {}
+    shape = {k: choice[k] for k in ("sort_tile", "traverse_block", "traverse_waves_per_cu", "streams")
+             if k in choice} if choice else {}
     srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10, **shape)
     t0 = srv.now()
     i, done = 0, []

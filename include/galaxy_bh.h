@@ -242,9 +242,9 @@ typedef struct {
     uint32_t traverse_block; /* traversal workgroup size: 64, 128, 256 */
     uint32_t traverse_waves_per_cu; /* persistent traversal wavefronts per CU, 1..32 */
     uint32_t use_graph;      /* capture each step in a HIP graph */
-    uint32_t streams;        /* batches: 1 or 2 groups of simulations, each stepped on a
-                                stream of its own so their kernels overlap (0 = 1);
-                                servers use one */
+    uint32_t streams;        /* 1 or 2 groups of simulations (a server: of in-flight
+                                requests), each stepped on a stream of its own so their
+                                kernels overlap (0 = 1) */
 } gbh_tuning;
 
 /* counts[nsims]; states = all simulations' records concatenated in order. */

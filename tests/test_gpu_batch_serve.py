@@ -98,11 +98,12 @@ def test_batch_many_sims_per_xcd_work(oracle, gpu, use_graph, block, waves, stre
     b.close()
 
 
-def test_server_results_same_alone_and_in_mix(oracle, gpu):
+@pytest.mark.parametrize("streams", [1, 2])
+def test_server_results_same_alone_and_in_mix(oracle, gpu, streams):
     sims = mixed_sims()
     steps = [10, 30, 20, 10, 40]
-    print("server mix", flush=True)
-    srv = Server(gpu, hbm_budget_bytes=8 << 30, steps_per_round=10)
+    print(f"server mix streams={streams}", flush=True)
+    srv = Server(gpu, hbm_budget_bytes=8 << 30, steps_per_round=10, streams=streams)
     ids = [srv.submit(s, c, dt, k) for (s, c, dt), k in zip(sims, steps)]
     done = srv.drain()
     assert sorted(d.id for d in done) == sorted(ids)
