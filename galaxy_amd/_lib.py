@@ -121,7 +121,7 @@ def lib():
         "gbh_batch_read": ([vp, u32, vp, vp], i32),
         "gbh_batch_force_evidence": ([vp, u32, vp, vp], i32),
         "gbh_batch_device_bytes": ([vp], u64),
-        "gbh_batch_last_phase_ms": ([vp, vp, u32, vp], i32),
+        "gbh_batch_profile_step": ([vp, vp, u32, vp], i32),
         "gbh_sim_device_bytes": ([u32], u64),
         "gbh_server_create": ([vp, vp, vp], i32),
         "gbh_server_destroy": ([vp], None),

@@ -76,6 +76,6 @@ class Batch:
         """Kernel time of one un-captured step per phase (advances the ensemble by one step)."""
         ms = np.zeros(8)
         k = C.c_uint32()
-        _check(L.lib().gbh_batch_last_phase_ms(self._h, _p(ms), 8, C.byref(k)))
+        _check(L.lib().gbh_batch_profile_step(self._h, _p(ms), 8, C.byref(k)))
         names = ["kick_drift_bounds", "keys", "radix_sort", "topology", "top_aggregates", "traversal"]
         return dict(zip(names, ms[: k.value].tolist()))

@@ -518,13 +518,15 @@ __device__ __forceinline__ void mix64(unsigned long long& h, unsigned long long 
 // ----------------------------------------------------------------------------
 // K1: kick/drift (+ optional preceding final kick) fused with the bounds partials
 // mode 0: bounds only; 1: kick/drift then bounds; 2: final kick + kick/drift then bounds
-// (nbody_evolve.wgsl:19-46, nbody_tree_parallel_bounds.wgsl:51-76)
+// (nbody_evolve.wgsl:19-46, nbody_tree_parallel_bounds.wgsl:51-76).
+// clear_errors: also clear the set's and every simulation's sticky error word
+// (the first build of a call; no kernel of this launch sets them).
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ sims,
                                                   const TileRef* __restrict__ tiles,
                                                   uint32_t* __restrict__ ctl,
                                                   unsigned long long* __restrict__ tflags,
-                                                  uint32_t ntopo, int mode) {
+                                                  uint32_t ntopo, int mode, uint32_t clear_errors) {
     __shared__ float s_red[4][kTile / 64];
     const TileRef tr = tiles[blockIdx.x];
     const SimDesc& S = sim_at(sims, tr.sim);
@@ -545,7 +547,9 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
     if (tr.base == 0) {
         for (uint32_t k = threadIdx.x; k < 4 * 256; k += kTile) S.meta->hist[k] = 0u;
         if (threadIdx.x < 8) S.meta->ticket[threadIdx.x] = 0u;
+        if (clear_errors && threadIdx.x == 0) S.meta->err = 0u;
     }
+    if (clear_errors && blockIdx.x == 0 && threadIdx.x == 0) ctl[kCtlError] = 0u;
     float x = 0.f, y = 0.f;
     if (valid) {
         if (mode != 0) {
@@ -595,6 +599,15 @@ __global__ __launch_bounds__(kTile) void k_bounds(const SimDesc* __restrict__ si
 
 __global__ void k_reset_queues(uint32_t* __restrict__ ctl) {
     if (threadIdx.x < kXcds) ctl[kCtlQueue0 + threadIdx.x * kQueueStride] = 0u;
+}
+
+// Clears the sticky error words of a set and of each of its simulations (the
+// start of a batched or served call: each call reports its own errors).
+__global__ __launch_bounds__(256) void k_clear_errors(const SimDesc* __restrict__ sims, uint32_t nsims,
+                                                      uint32_t* __restrict__ ctl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) ctl[kCtlError] = 0u;
+    if (i < nsims && sims[i].meta) sims[i].meta->err = 0u;
 }
 
 // Sticky error words of a set into one array: out[0] = the set's control error

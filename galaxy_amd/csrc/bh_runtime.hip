@@ -288,9 +288,16 @@ struct SimSet {
     }
 
     // ---- launches ----
-    void bounds(int mode, hipStream_t s) {
+    void bounds(int mode, hipStream_t s, bool clear_errors = false) {
         hipLaunchKernelGGL(k_bounds, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc,
-                           d_t256, d_ctl, d_tflags, (uint32_t)ttopo.size(), mode);
+                           d_t256, d_ctl, d_tflags, (uint32_t)ttopo.size(), mode, clear_errors ? 1u : 0u);
+        HIP_OK(hipGetLastError());
+    }
+    // Sticky error words of the set and its simulations back to zero (a kernel,
+    // so it may also run where a caller captures the stream).
+    void clear_errors(hipStream_t s) {
+        const uint32_t nsims = (uint32_t)desc.size();
+        hipLaunchKernelGGL(k_clear_errors, dim3(div_up(nsims, 256u)), dim3(256), 0, s, d_desc, nsims, d_ctl);
         HIP_OK(hipGetLastError());
     }
     void keys(hipStream_t s) {
@@ -946,8 +953,10 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
         hipStream_t s = rt->dev->stream;
         if (memcmp(&prev, &d, sizeof d) != 0) t->set.upload_desc(s);
         double t0 = now_seconds();
+        // a failed rebuild leaves no usable tree until the next one succeeds
+        t->built = false;
         HIP_OK(hipEventRecord(t->ev[0], s));
-        t->set.bounds(0, s);
+        t->set.bounds(0, s, true);  // also clears the error words of the previous rebuild
         t->queue_zero = true;
         HIP_OK(hipEventRecord(t->ev[1], s));
         t->set.keys(s);

@@ -258,8 +258,11 @@ int gbh_batch_step(gbh_batch* b, uint32_t steps);
 int gbh_batch_read(gbh_batch* b, uint32_t sim, GbhEvolveState* out, GbhGpuAccel* accel);
 int gbh_batch_force_evidence(gbh_batch* b, uint32_t sim, uint32_t* stats, uint64_t* hash);
 uint64_t gbh_batch_device_bytes(gbh_batch* b);
-/* Kernel time of the last gbh_batch_step per phase (ms, HIP events), for profiling. */
-int gbh_batch_last_phase_ms(gbh_batch* b, double* ms, uint32_t cap, uint32_t* count);
+/* Profiling: runs ONE MORE KDK step of every simulation, un-captured and with
+ * each phase's launches in sequence, and writes its kernel time per phase (ms,
+ * HIP events): kick/drift + bounds, keys, sort, topology, top aggregates,
+ * traversal.  The ensemble advances by that step, like a gbh_batch_step(b, 1). */
+int gbh_batch_profile_step(gbh_batch* b, double* ms, uint32_t cap, uint32_t* count);
 
 /* ---------------------------------------------------------------------------
  * Serving (no reference counterpart): continuous batching of "advance this

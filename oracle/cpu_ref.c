@@ -513,80 +513,112 @@ static void point_mass_f32(float tx, float ty, float sx, float sy, float mass, f
 }
 
 /* bh_traverse (nbody.wgsl:43-123) */
+/* One target of bh_traverse (nbody.wgsl:43-123): returns 0, or -1 with the
+ * error set. */
+static int traverse_one_f32(const cref_bh_settings* st, const cref_gpu_body* bodies,
+                            const cref_gpu_entry* entries, const cref_gpu_cell* cells,
+                            uint32_t t, cref_gpu_accel* out, uint32_t* stats, uint64_t* hash) {
+    uint32_t ncells = st->info[1];
+    float theta = st->physics[0], eps = st->physics[1], g = st->physics[2];
+    float tx = bodies[t].position_mass[0];
+    float ty = bodies[t].position_mass[1];
+    uint32_t tpos = bodies[t].meta[0];
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t stack[64];
+    uint32_t len = 1;
+    stack[0] = st->info[2];
+    uint32_t visited = 0, accepted = 0, direct = 0;
+    uint64_t h = FNV_OFFSET;
+    while (len > 0) {
+        uint32_t ci = stack[--len];
+        if (ci >= ncells) { set_error("cell index out of range"); return -1; }
+        const cref_gpu_cell* c = cells + ci;
+        visited += 1;
+        int leaf = c->children[0] == CREF_NO_CHILD && c->children[1] == CREF_NO_CHILD &&
+                   c->children[2] == CREF_NO_CHILD && c->children[3] == CREF_NO_CHILD;
+        if (leaf) {
+            for (uint32_t p = c->range_depth[0]; p < c->range_depth[1]; ++p) {
+                uint32_t other = entries[p].data[1];
+                if (other != t) {
+                    float px, py;
+                    const float* o = bodies[other].position_mass;
+                    point_mass_f32(tx, ty, o[0], o[1], o[2], eps, g, &px, &py);
+                    ax = ax + px;
+                    ay = ay + py;
+                    direct += 1;
+                    checksum_word(&h, (1ull << 60) | other);
+                }
+            }
+            continue;
+        }
+        float dx = c->com[0] - tx;
+        float dy = c->com[1] - ty;
+        float d2 = dx * dx + dy * dy;
+        int contains = tpos >= c->range_depth[0] && tpos < c->range_depth[1];
+        if (!contains && d2 > 0.0f) {
+            float distance = sqrtf(d2);
+            float w = 2.0f * c->center_half_mass[2];
+            if (w / distance < theta) {
+                float px, py;
+                point_mass_f32(tx, ty, c->com[0], c->com[1], c->center_half_mass[3], eps, g,
+                               &px, &py);
+                ax = ax + px;
+                ay = ay + py;
+                accepted += 1;
+                checksum_word(&h, (2ull << 60) | ((uint64_t)c->range_depth[2] << 32) |
+                                      c->range_depth[0]);
+                continue;
+            }
+        }
+        for (int q = 3; q >= 0; --q) {
+            if (c->children[q] != CREF_NO_CHILD) {
+                if (len >= 64) { set_error("traversal stack overflow"); return -1; }
+                stack[len++] = c->children[q];
+            }
+        }
+    }
+    out[t].value[0] = ax;
+    out[t].value[1] = ay;
+    out[t].value[2] = 0.0f;
+    out[t].value[3] = 0.0f;
+    if (stats) {
+        stats[3 * (size_t)t + 0] = visited;
+        stats[3 * (size_t)t + 1] = accepted;
+        stats[3 * (size_t)t + 2] = direct;
+    }
+    if (hash) hash[t] = h;
+    return 0;
+}
+
+/* Threads of the f32 traversal (cref_set_threads); 1 by default.  Targets are
+ * independent, so every thread count gives identical results. */
+static int g_threads = 1;
+
+void cref_set_threads(int threads) { g_threads = threads > 0 ? threads : 1; }
+
 int cref_traverse_f32(const cref_bh_settings* st, const cref_gpu_body* bodies,
                       const cref_gpu_entry* entries, const cref_gpu_cell* cells,
                       cref_gpu_accel* out, uint32_t* stats, uint64_t* hash) {
-    uint32_t n = st->info[0];
-    uint32_t ncells = st->info[1];
-    float theta = st->physics[0], eps = st->physics[1], g = st->physics[2];
-    for (uint32_t t = 0; t < n; ++t) {
-        float tx = bodies[t].position_mass[0];
-        float ty = bodies[t].position_mass[1];
-        uint32_t tpos = bodies[t].meta[0];
-        float ax = 0.0f, ay = 0.0f;
-        uint32_t stack[64];
-        uint32_t len = 1;
-        stack[0] = st->info[2];
-        uint32_t visited = 0, accepted = 0, direct = 0;
-        uint64_t h = FNV_OFFSET;
-        while (len > 0) {
-            uint32_t ci = stack[--len];
-            if (ci >= ncells) { set_error("cell index out of range"); return -1; }
-            const cref_gpu_cell* c = cells + ci;
-            visited += 1;
-            int leaf = c->children[0] == CREF_NO_CHILD && c->children[1] == CREF_NO_CHILD &&
-                       c->children[2] == CREF_NO_CHILD && c->children[3] == CREF_NO_CHILD;
-            if (leaf) {
-                for (uint32_t p = c->range_depth[0]; p < c->range_depth[1]; ++p) {
-                    uint32_t other = entries[p].data[1];
-                    if (other != t) {
-                        float px, py;
-                        const float* o = bodies[other].position_mass;
-                        point_mass_f32(tx, ty, o[0], o[1], o[2], eps, g, &px, &py);
-                        ax = ax + px;
-                        ay = ay + py;
-                        direct += 1;
-                        checksum_word(&h, (1ull << 60) | other);
-                    }
-                }
-                continue;
-            }
-            float dx = c->com[0] - tx;
-            float dy = c->com[1] - ty;
-            float d2 = dx * dx + dy * dy;
-            int contains = tpos >= c->range_depth[0] && tpos < c->range_depth[1];
-            if (!contains && d2 > 0.0f) {
-                float distance = sqrtf(d2);
-                float w = 2.0f * c->center_half_mass[2];
-                if (w / distance < theta) {
-                    float px, py;
-                    point_mass_f32(tx, ty, c->com[0], c->com[1], c->center_half_mass[3], eps, g,
-                                   &px, &py);
-                    ax = ax + px;
-                    ay = ay + py;
-                    accepted += 1;
-                    checksum_word(&h, (2ull << 60) | ((uint64_t)c->range_depth[2] << 32) |
-                                          c->range_depth[0]);
-                    continue;
-                }
-            }
-            for (int q = 3; q >= 0; --q) {
-                if (c->children[q] != CREF_NO_CHILD) {
-                    if (len >= 64) { set_error("traversal stack overflow"); return -1; }
-                    stack[len++] = c->children[q];
-                }
+    const int64_t n = st->info[0];
+    int failed = 0;
+    char first_error[sizeof g_error] = {0};
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+    for (int64_t t = 0; t < n; ++t) {
+        int stop;
+#pragma omp atomic read
+        stop = failed;
+        if (stop) continue;
+        if (traverse_one_f32(st, bodies, entries, cells, (uint32_t)t, out, stats, hash) != 0) {
+#pragma omp critical
+            {
+                if (!failed) memcpy(first_error, g_error, sizeof first_error);
+                failed = 1;
             }
         }
-        out[t].value[0] = ax;
-        out[t].value[1] = ay;
-        out[t].value[2] = 0.0f;
-        out[t].value[3] = 0.0f;
-        if (stats) {
-            stats[3 * (size_t)t + 0] = visited;
-            stats[3 * (size_t)t + 1] = accepted;
-            stats[3 * (size_t)t + 2] = direct;
-        }
-        if (hash) hash[t] = h;
+    }
+    if (failed) {
+        memcpy(g_error, first_error, sizeof g_error);
+        return -1;
     }
     return 0;
 }

@@ -105,6 +105,8 @@ int cref_validate_parallel_order(uint32_t n, const cref_gpu_entry* entries);
 int cref_traverse_f32(const cref_bh_settings* settings, const cref_gpu_body* bodies,
                       const cref_gpu_entry* entries, const cref_gpu_cell* cells,
                       cref_gpu_accel* out, uint32_t* stats, uint64_t* hash);
+/* Threads the f32 traversal above uses (default 1; results do not depend on it). */
+void cref_set_threads(int threads);
 /* f64 flat traversal (flat.rs:270-371); stats_total = {visited, approximated, direct} */
 int cref_flat_accelerations(const cref_tree* t, uint32_t n, const double* x, const double* y,
                             const double* m, const cref_config* cfg, double* ax, double* ay,

@@ -73,6 +73,7 @@ def lib():
             "cref_parallel_tree_checksum": ([vp, u32, vp, u32, vp], u64),
             "cref_validate_parallel_order": ([u32, vp], i32),
             "cref_traverse_f32": ([vp, vp, vp, vp, vp, vp, vp], i32),
+            "cref_set_threads": ([i32], None),
             "cref_flat_accelerations": ([vp, u32, vp, vp, vp, cfgp, vp, vp, vp], i32),
             "cref_recursive_accelerations": ([u32, vp, vp, vp, cfgp, vp, vp, vp, vp], i32),
             "cref_direct_accelerations": ([u32, vp, vp, vp, cfgp, vp, vp], i32),
@@ -110,6 +111,11 @@ def _check(rc):
 
 def threads_default():
     return max(1, min(16, os.cpu_count() or 1))
+
+
+def set_threads(threads):
+    """Threads of the f32 traversal (every count gives identical results)."""
+    lib().cref_set_threads(int(threads))
 
 
 class FlatTree:

@@ -211,6 +211,34 @@ def test_packed_ingest_sparse_layout(oracle, gpu):
     assert np.array_equal(got.view(np.uint32), oracle.traverse_f32(st, bodies, entries, cells).view(np.uint32))
 
 
+def test_packed_ingest_rejects_overlapping_leaves_beyond_node_cap(gpu):
+    """A malformed host tree: 87,383 cells in level order (cell k's children are
+    4k+1..4k+4), every cell covering all 65,536 bodies.  Its 65,537 leaves each
+    expand to 65,536 stream nodes: 4,295,032,832 + 21,846 internal nodes, which a
+    32-bit count wraps to 87,382.  The host rejects the tree before sizing or
+    launching anything."""
+    n, ncells = 65536, 87383
+    st = np.zeros(8, np.uint32)
+    st[0], st[1], st[2] = n, ncells, 0
+    st[4:7] = np.array([0.5, 0.01, 1.0], np.float32).view(np.uint32)
+    bodies = np.zeros((n, 8), np.uint32)
+    bodies[:, 2] = np.float32(1.0 / n).view(np.uint32)
+    bodies[:, 4] = np.arange(n, dtype=np.uint32)
+    entries = np.zeros((n, 4), np.uint32)
+    entries[:, 1] = np.arange(n, dtype=np.uint32)
+    cells = np.zeros((ncells, 16), np.uint32)
+    cells[:, 2] = np.float32(1.0).view(np.uint32)
+    cells[:, 3] = np.float32(1.0).view(np.uint32)
+    cells[:, 9] = n
+    child = 4 * np.arange(ncells, dtype=np.int64)[:, None] + np.arange(1, 5)[None, :]
+    cells[:, 12:16] = np.where(child < ncells, child, 0xFFFFFFFF).astype(np.uint32)
+    leaves = int((cells[:, 12:16] == 0xFFFFFFFF).all(1).sum())
+    total = leaves * n + (ncells - leaves)
+    assert leaves == 65537 and total >= 1 << 32 and total % (1 << 32) < 1 << 26
+    with pytest.raises(BhError, match="too many nodes"):
+        gpu.traverse(st, bodies, entries, cells)
+
+
 def test_beyond_cap_tree(oracle, gpu, rt):
     s = ics.binary_disc(100000, 100000, softening=0.002)
     cfg = Config(theta=0.6, softening_kpc=0.002, g=1.0, bucket=4, max_depth=16)

@@ -1,0 +1,213 @@
+"""GPU parity at the measured workloads' full sizes.
+
+- C3: the 256-simulation ensemble (3,436,034 bodies) at the bench's launch shape
+  (two groups on two streams, HIP-graph steps): the initial force and two steps,
+  every simulation's state and acceleration bit-exact against the oracle's f32
+  trajectory, and every simulation's per-target interaction counts and
+  interaction-list hashes against the oracle's traversal of its final tree.
+- C2: the 1,048,576-body binary disc encounter: tree (keys, permutation, every
+  cell record, sparse evidence and checksum) and forces with interaction
+  evidence, bit-exact.
+- C4: a server mix whose requests include 70,000 and 262,144 bodies among small
+  ones; each result equals the oracle's trajectory and the same request run
+  alone.
+- C1: ten steps of the 65,536-body drop-in loop (galaxy-bh-gpu-tree-parallel.rs:
+  341-350) with every step's sparse evidence, its checksum
+  (:150-187) and order (:189-203), and its forces compared with the oracle's
+  tree of the GPU's own state (nbody/src/flat.rs:217-268), then the repeat
+  rebuild check (:356-366) and the whole trajectory against the oracle's.
+
+The oracle's f32 traversal runs on several threads here (its results do not
+depend on the thread count); ensembles run one simulation per thread.
+"""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from galaxy_amd import Config, NbodyGpu, ParallelTreeRuntime, ics
+from galaxy_amd.batch import Batch
+from galaxy_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    g = NbodyGpu()
+    yield g
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def rt(gpu):
+    return ParallelTreeRuntime(gpu)
+
+
+def ocfg(oracle, c: Config):
+    return oracle.make_config(theta=c.theta, softening=c.softening_kpc, g=c.g, bucket=c.bucket,
+                              max_depth=c.max_depth)
+
+
+def same_bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+
+
+def test_c3_ensemble_256_at_bench_shape(oracle):
+    gpu = NbodyGpu()
+    w = ics.config3()
+    assert len(w.states) == 256 and w.bodies == 3436034
+    cfg = w.config
+    print(f"C3 ensemble: {len(w.states)} sims, {w.bodies} bodies, streams=2, graph", flush=True)
+    b = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=2048, traverse_block=256,
+              traverse_waves_per_cu=16, streams=2)
+    b.step(1)
+    b.step(1)
+    got = [b.read(i) for i in range(len(w.states))]
+    print("  evidence of every simulation", flush=True)
+    evid = [b.force_evidence(i) for i in range(len(w.states))]
+    b.close()
+    c = ocfg(oracle, cfg)
+    oracle.set_threads(1)
+
+    def ref(i):
+        s, a, st, h = oracle.f32_evolve(w.states[i], 2, w.dt, c, evidence=True)
+        return s, a, st, h
+
+    print("  oracle trajectories", flush=True)
+    with ThreadPoolExecutor(THREADS) as ex:
+        refs = list(ex.map(ref, range(len(w.states))))
+    bad = [i for i in range(len(w.states))
+           if not (same_bits(got[i][0], refs[i][0]) and same_bits(got[i][1], refs[i][1]))]
+    assert not bad, f"simulations differing from the oracle: {bad[:10]}"
+    bad_ev = [i for i in range(len(w.states))
+              if not (np.array_equal(evid[i][0], refs[i][2]) and np.array_equal(evid[i][1], refs[i][3]))]
+    assert not bad_ev, f"interaction evidence differs: {bad_ev[:10]}"
+    inter = sum(int(e[0][:, 1].sum() + e[0][:, 2].sum()) for e in evid)
+    print(f"  {inter} interactions per force state, all bit-exact", flush=True)
+
+
+def test_c2_full_size_tree_and_force(oracle, gpu, rt):
+    w = ics.config2()
+    s = w.states[0]
+    assert len(s) == 1048576
+    cfg = w.config
+    print(f"C2 tree + force: {len(s)} bodies, theta {cfg.theta}", flush=True)
+    ev = gpu.create_evolving_state(s)
+    tree = rt.create_tree(gpu, ev, large=True)
+    rt.rebuild(gpu, ev, tree, cfg)
+    rt.force(gpu, ev, tree, cfg)
+    acc = gpu.read_evolving_accelerations(ev)
+    g_entries, g_cells, g_pos = rt.read_flat(tree)
+    g_ev = rt.read_evidence(gpu, tree)
+    g_stats, g_hash = rt.force_evidence(ev, tree, cfg)
+    tree.close()
+    ev.close()
+    c = ocfg(oracle, cfg)
+    t = oracle.tree_from_state(s, c)
+    st, bodies, entries, cells = t.pack()
+    assert np.array_equal(g_entries, entries), "Morton keys / sort permutation differ"
+    assert np.array_equal(g_pos, bodies[:, 4]), "target positions differ"
+    assert g_cells.shape == cells.shape
+    bad = np.flatnonzero((g_cells != cells).any(1))
+    assert bad.size == 0, f"{bad.size} cells differ, first {bad[:5]}"
+    meta, s_entries, s_cells = t.sparse_evidence()
+    assert np.array_equal(g_ev.meta, meta) and np.array_equal(g_ev.entries, s_entries)
+    assert np.array_equal(g_ev.cells, s_cells)
+    assert oracle.parallel_tree_checksum(g_ev.meta, g_ev.entries, g_ev.cells) == \
+        oracle.parallel_tree_checksum(meta, s_entries, s_cells)
+    assert oracle.validate_parallel_order(g_ev.entries)
+    oracle.set_threads(THREADS)
+    ref, stats, hsh = oracle.traverse_f32(st, bodies, entries, cells, evidence=True)
+    oracle.set_threads(1)
+    assert same_bits(acc, ref), f"max |diff| {np.abs(acc - ref).max()}"
+    assert np.array_equal(g_stats, stats) and np.array_equal(g_hash, hsh)
+    print(f"  {len(cells)} cells, {int(stats[:, 1:].sum())} interactions, bit-exact", flush=True)
+
+
+def test_c4_server_large_requests_alone_and_in_mix(oracle, gpu):
+    disc = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    coll = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)
+    reqs = [
+        (ics.uniform_collapse(262144, 10**6 + 1), coll, 5e-4, 10),
+        (ics.rotating_disc(1024, 10**6 + 2, softening=0.01), disc, 1e-3, 30),
+        (ics.rotating_disc(70000, 10**6 + 3, softening=0.01), disc, 1e-3, 20),
+        (ics.uniform_collapse(3000, 10**6 + 4), coll, 5e-4, 10),
+        (ics.rotating_disc(40000, 10**6 + 5, softening=0.01), disc, 1e-3, 10),
+    ]
+    print("C4 server mix: 262144, 1024, 70000, 3000, 40000 bodies; streams=2", flush=True)
+    srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10, streams=2,
+                 traverse_waves_per_cu=16)
+    ids = [srv.submit(*r[:3], r[3]) for r in reqs[:3]]
+    srv.round()                                      # the first three are in flight
+    ids += [srv.submit(*r[:3], r[3]) for r in reqs[3:]]  # two more arrive meanwhile
+    done = srv.drain()
+    assert sorted(d.id for d in done) == sorted(ids) and all(d.status == 0 for d in done)
+    mixed = [srv.take(i) for i in ids]
+    srv.close()
+    print("  the largest request alone", flush=True)
+    alone = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10)
+    j = alone.submit(*reqs[0][:3], reqs[0][3])
+    alone.drain()
+    s_alone, a_alone = alone.take(j)
+    alone.close()
+    assert same_bits(s_alone, mixed[0][0]) and same_bits(a_alone, mixed[0][1])
+    oracle.set_threads(1)
+
+    def ref(k):
+        s0, c, dt, steps = reqs[k]
+        return oracle.f32_evolve(s0, steps, dt, ocfg(oracle, c))
+
+    print("  oracle trajectories", flush=True)
+    with ThreadPoolExecutor(len(reqs)) as ex:
+        refs = list(ex.map(ref, range(len(reqs))))
+    for k in range(len(reqs)):
+        assert same_bits(mixed[k][0], refs[k][0]), k
+        assert same_bits(mixed[k][1], refs[k][1]), k
+
+
+def test_c1_ten_steps_checked_every_step(oracle, gpu, rt):
+    w = ics.config1()
+    s0 = w.states[0]
+    cfg, dt = w.config, w.dt
+    c = ocfg(oracle, cfg)
+    oracle.set_threads(THREADS)
+    ev = gpu.create_evolving_state(s0)
+    tree = rt.create_tree(gpu, ev)
+
+    def check_build_and_force(label):
+        state = gpu.read_evolving_state(ev)
+        g_ev = rt.read_evidence(gpu, tree)
+        assert oracle.validate_parallel_order(g_ev.entries), label
+        t = oracle.tree_from_state(state, c)
+        meta, s_entries, s_cells = t.sparse_evidence()
+        g_sum = oracle.parallel_tree_checksum(g_ev.meta, g_ev.entries, g_ev.cells)
+        assert g_sum == oracle.parallel_tree_checksum(meta, s_entries, s_cells), label
+        assert np.array_equal(g_ev.cells, s_cells) and np.array_equal(g_ev.entries, s_entries), label
+        rt.force(gpu, ev, tree, cfg)
+        ref = oracle.traverse_f32(*t.pack())
+        assert same_bits(gpu.read_evolving_accelerations(ev), ref), label
+        return g_sum
+
+    print("C1 drop-in: 65536 bodies, 10 steps, every step checked", flush=True)
+    rt.rebuild(gpu, ev, tree, cfg)
+    check_build_and_force("initial")
+    for k in range(10):
+        gpu.kick_drift(ev, dt)
+        rt.rebuild(gpu, ev, tree, cfg)
+        final_sum = check_build_and_force(f"step {k + 1}")
+        gpu.final_kick(ev, dt)
+    rt.rebuild(gpu, ev, tree, cfg)  # unchanged state: the same tree
+    g_ev = rt.read_evidence(gpu, tree)
+    assert oracle.parallel_tree_checksum(g_ev.meta, g_ev.entries, g_ev.cells) != 0
+    s = gpu.read_evolving_state(ev)
+    rs, ra = oracle.f32_evolve(s0, 10, dt, c)
+    oracle.set_threads(1)
+    assert same_bits(s, rs)
+    assert same_bits(gpu.read_evolving_accelerations(ev), ra)
+    # the repeat rebuild reproduces the tree of the last force state when the
+    # state has not moved since (the final kick changes velocities only)
+    assert oracle.parallel_tree_checksum(g_ev.meta, g_ev.entries, g_ev.cells) == final_sum
