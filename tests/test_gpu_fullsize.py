@@ -142,9 +142,9 @@ def test_c4_server_large_requests_alone_and_in_mix(oracle, gpu):
     srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10, streams=2,
                  traverse_waves_per_cu=16)
     ids = [srv.submit(*r[:3], r[3]) for r in reqs[:3]]
-    srv.round()                                      # the first three are in flight
+    done = srv.round()                               # the first three are in flight
     ids += [srv.submit(*r[:3], r[3]) for r in reqs[3:]]  # two more arrive meanwhile
-    done = srv.drain()
+    done += srv.drain()
     assert sorted(d.id for d in done) == sorted(ids) and all(d.status == 0 for d in done)
     mixed = [srv.take(i) for i in ids]
     srv.close()
