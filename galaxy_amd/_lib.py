@@ -95,6 +95,7 @@ def lib():
         "gbh_tree_rebuild": ([vp, vp, vp, vp, vp], i32),
         "gbh_force": ([vp, vp, vp, vp, vp], i32),
         "gbh_force_evidence": ([vp, vp, vp, vp, vp, vp], i32),
+        "gbh_force_evidence_noalloc": ([vp, vp, vp, vp, vp, vp], i32),
         "gbh_tree_counts": ([vp, vp, vp, vp], i32),
         "gbh_tree_read_evidence": ([vp, vp, vp, vp, u32, vp, vp, vp], i32),
         "gbh_tree_read_flat": ([vp, vp, vp, u32, vp, vp], i32),

@@ -301,6 +301,15 @@ class ParallelTreeRuntime:
                                           _p(stats), _p(hsh)))
         return stats, hsh
 
+    def force_evidence_noalloc(self, evolving: EvolvingState, tree: ParallelGpuTree, config: Config):
+        """force_evidence through the entry point that allocates no device memory
+        (evidence staged in the library's per-GPU scratch array; at most 65,536 bodies)."""
+        stats = np.zeros((evolving.count, 3), np.uint32)
+        hsh = np.zeros(evolving.count, np.uint64)
+        _check(L.lib().gbh_force_evidence_noalloc(self._h, evolving._h, tree._h, C.byref(config.c()),
+                                                  _p(stats), _p(hsh)))
+        return stats, hsh
+
     def counts(self, tree: ParallelGpuTree):
         a, lf, md = C.c_uint32(), C.c_uint32(), C.c_uint32()
         _check(L.lib().gbh_tree_counts(tree._h, C.byref(a), C.byref(lf), C.byref(md)))

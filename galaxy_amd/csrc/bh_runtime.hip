@@ -137,7 +137,26 @@ struct gbh_device {
     hipStream_t stream = nullptr;
     std::string info;
     int cu_count = 0;
+    // Device address of this GPU's copy of `scratch` (below), from
+    // hipGetSymbolAddress with the device current; the array's name in host
+    // code is not a device address.
+    unsigned char* scratch = nullptr;
 };
+
+namespace gbh {
+
+// Scratch for the entry points that allocate nothing (gbh_force_evidence_noalloc):
+// the per-target interaction evidence of up to kScratchTargets targets, staged
+// here between the traversal and the copy to the caller.  One copy per GPU, held
+// until the process ends; every call on a GPU shares it, so calls that overlap
+// on one GPU race.
+constexpr uint32_t kScratchTargets = GBH_NOALLOC_MAX_BODIES;
+constexpr size_t kScratchStatsBytes = 12ull * kScratchTargets;
+constexpr size_t kScratchBytes = kScratchStatsBytes + 8ull * kScratchTargets;
+static_assert(kScratchStatsBytes % 256 == 0, "the hash part of the scratch stays 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+}  // namespace gbh
 
 // ============================================================================
 // SimSet: the simulations one group of launches advances
@@ -682,9 +701,13 @@ int gbh_device_open(const char* selector, int allow_software, gbh_device** out) 
             return -1;
         }
         HIP_OK(hipSetDevice(index));
+        // this device's copy of the no-allocation scratch (the device is current)
+        unsigned char* scratch_ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress((void**)&scratch_ptr, HIP_SYMBOL(gbh::scratch)));
         auto* d = new gbh_device();
         d->index = index;
         d->count = count;
+        d->scratch = scratch_ptr;
         HIP_OK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
         hipDeviceProp_t p;
         HIP_OK(hipGetDeviceProperties(&p, index));
@@ -988,11 +1011,23 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
     });
 }
 
+// noalloc: stage the evidence in the device's scratch array instead of device
+// memory allocated for the call (n <= GBH_NOALLOC_MAX_BODIES).
 static int force_impl(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_config* cfg,
-                      gbh_stage_timing* timing, uint32_t* stats, uint64_t* hash) {
+                      gbh_stage_timing* timing, uint32_t* stats, uint64_t* hash,
+                      bool noalloc = false) {
     return guarded([&] {
         if (!rt || !st || !t) {
             set_error("null argument");
+            return -1;
+        }
+        if (noalloc && t->n > kScratchTargets) {
+            set_error("gbh_force_evidence_noalloc supports at most " + std::to_string(kScratchTargets) +
+                      " bodies; this tree has " + std::to_string(t->n));
+            return -1;
+        }
+        if (noalloc && !rt->dev->scratch) {
+            set_error("the device has no scratch address");
             return -1;
         }
         if (t->n != st->n || t->st != st) {
@@ -1027,10 +1062,25 @@ static int force_impl(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_con
             HIP_OK(hipGetLastError());
         }
         double t1 = now_seconds();
+        // device staging of the evidence: allocated for this call (freed on every
+        // exit), or the device's scratch array
+        struct Owned {
+            void* p = nullptr;
+            ~Owned() {
+                if (p) (void)hipFree(p);
+            }
+        } own_stats, own_hash;
         uint32_t* d_stats = nullptr;
         unsigned long long* d_hash = nullptr;
-        if (stats) HIP_OK(hipMalloc((void**)&d_stats, 12ull * t->n));
-        if (hash) HIP_OK(hipMalloc((void**)&d_hash, 8ull * t->n));
+        if (noalloc) {
+            if (stats) d_stats = (uint32_t*)rt->dev->scratch;
+            if (hash) d_hash = (unsigned long long*)(rt->dev->scratch + kScratchStatsBytes);
+        } else {
+            if (stats) HIP_OK(hipMalloc(&own_stats.p, 12ull * t->n));
+            if (hash) HIP_OK(hipMalloc(&own_hash.p, 8ull * t->n));
+            d_stats = (uint32_t*)own_stats.p;
+            d_hash = (unsigned long long*)own_hash.p;
+        }
         if (d_stats || d_hash) t->set.fill_body_cold(s);
         HIP_OK(hipEventRecord(t->ev[6], s));
         t->set.force(s, !t->queue_zero, d_stats, d_hash);
@@ -1039,10 +1089,9 @@ static int force_impl(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_con
         HIP_OK(hipStreamSynchronize(s));
         double t2 = now_seconds();
         t->last_force_ms = ms_between(t->ev[6], t->ev[7]);
-        if (stats) HIP_OK(hipMemcpy(stats, d_stats, 12ull * t->n, hipMemcpyDeviceToHost));
-        if (hash) HIP_OK(hipMemcpy(hash, d_hash, 8ull * t->n, hipMemcpyDeviceToHost));
-        if (d_stats) HIP_OK(hipFree(d_stats));
-        if (d_hash) HIP_OK(hipFree(d_hash));
+        if (stats) HIP_OK(hipMemcpyAsync(stats, d_stats, 12ull * t->n, hipMemcpyDeviceToHost, s));
+        if (hash) HIP_OK(hipMemcpyAsync(hash, d_hash, 8ull * t->n, hipMemcpyDeviceToHost, s));
+        if (stats || hash) HIP_OK(hipStreamSynchronize(s));
         if (timing) *timing = gbh_stage_timing{t1 - t0, t2 - t1, 0};
         return 0;
     });
@@ -1060,6 +1109,15 @@ int gbh_force_evidence(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_co
         return -1;
     }
     return force_impl(rt, st, t, cfg, nullptr, stats, hash);
+}
+
+int gbh_force_evidence_noalloc(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_config* cfg,
+                               uint32_t* stats, uint64_t* hash) {
+    if (!stats && !hash) {
+        set_error("evidence outputs are both null");
+        return -1;
+    }
+    return force_impl(rt, st, t, cfg, nullptr, stats, hash, true);
 }
 
 int gbh_tree_counts(gbh_tree* t, uint32_t* active, uint32_t* leaves, uint32_t* max_depth) {

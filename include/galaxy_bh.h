@@ -146,6 +146,18 @@ int gbh_force(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* 
  * Either pointer may be NULL. */
 int gbh_force_evidence(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* cfg,
                        uint32_t* stats, uint64_t* hash);
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): gbh_force_evidence with the same arguments and results, but the
+ * per-target evidence is staged in a fixed device array of the library (one per
+ * GPU, 20 bytes per target, held until the process ends) instead of device
+ * memory allocated for the call.  The call allocates nothing.  Trees of more
+ * than GBH_NOALLOC_MAX_BODIES bodies are rejected with an error.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several host
+ * threads) race, and must be serialized by the caller. */
+#define GBH_NOALLOC_MAX_BODIES 65536u
+int gbh_force_evidence_noalloc(gbh_runtime* rt, gbh_state* st, gbh_tree* tree,
+                               const gbh_config* cfg, uint32_t* stats, uint64_t* hash);
 /* Cell statistics of the last rebuild: active cells, leaves, max depth. */
 int gbh_tree_counts(gbh_tree* tree, uint32_t* active, uint32_t* leaves, uint32_t* max_depth);
 /* read_evidence (nbody_parallel.rs:929-988): sparse level-order layout, cells in
