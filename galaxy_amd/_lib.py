@@ -122,6 +122,7 @@ def lib():
         "gbh_batch_read": ([vp, u32, vp, vp], i32),
         "gbh_batch_force_evidence": ([vp, u32, vp, vp], i32),
         "gbh_batch_device_bytes": ([vp], u64),
+        "gbh_batch_tree_counts": ([vp, u32, vp, vp], i32),
         "gbh_batch_profile_step": ([vp, vp, u32, vp], i32),
         "gbh_sim_device_bytes": ([u32], u64),
         "gbh_server_create": ([vp, vp, vp], i32),
@@ -129,6 +130,9 @@ def lib():
         "gbh_server_now": ([vp], dbl),
         "gbh_server_submit": ([vp, vp, u32, vp, u32, vp], i32),
         "gbh_server_round": ([vp, vp, u32, vp], i32),
+        "gbh_server_start": ([vp], i32),
+        "gbh_server_stop": ([vp], i32),
+        "gbh_server_poll": ([vp, vp, u32, vp, dbl], i32),
         "gbh_server_take_result": ([vp, u64, vp, vp], i32),
         "gbh_server_stats": ([vp, vp, vp, vp], i32),
         "gbh_ic_rotating_disc": ([u32, u64, dbl, dbl, dbl, dbl, dbl, dbl, vp], i32),

@@ -69,6 +69,12 @@ class Batch:
         _check(L.lib().gbh_batch_force_evidence(self._h, sim, _p(st), _p(h)))
         return st, h
 
+    def tree_counts(self, sim: int):
+        """(cells, traversal nodes) of a simulation's current tree."""
+        c, n = C.c_uint32(), C.c_uint32()
+        _check(L.lib().gbh_batch_tree_counts(self._h, sim, C.byref(c), C.byref(n)))
+        return int(c.value), int(n.value)
+
     def device_bytes(self) -> int:
         return int(L.lib().gbh_batch_device_bytes(self._h))
 

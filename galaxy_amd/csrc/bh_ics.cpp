@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/galaxy_bh.h"
 
@@ -49,14 +50,53 @@ struct XorShift64 {
 // Enclosed-mass fraction of an exponential disc at q = R / scale.
 double exp_disc_cdf(double q) { return 1.0 - (1.0 + q) * std::exp(-q); }
 
-// Inverse of exp_disc_cdf on [0, qmax] by fixed-count bisection (deterministic).
+// Starting points of the inverse below: q at targets k / kInvTable, k < kInvTable,
+// each by bisection on [0, 64] (computed once per process).
+constexpr int kInvTable = 4096;
+const double* inverse_table() {
+    static const std::vector<double> table = [] {
+        std::vector<double> t(kInvTable + 1);
+        for (int k = 0; k <= kInvTable; ++k) {
+            const double target = (double)k / kInvTable;
+            double lo = 0.0, hi = 64.0;
+            for (int it = 0; it < 64; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                if (exp_disc_cdf(mid) < target) lo = mid; else hi = mid;
+            }
+            t[k] = 0.5 * (lo + hi);
+        }
+        return t;
+    }();
+    return table.data();
+}
+
+// Inverse of exp_disc_cdf on [0, qmax]: linear interpolation in the table, then
+// Newton steps on cdf(q) - target (cdf'(q) = q e^-q), kept inside [0, qmax] by
+// bisecting toward the violated end; a fixed sequence of operations per target,
+// so the result depends only on the target and qmax.
 double exp_disc_radius(double target, double qmax) {
-    double lo = 0.0, hi = qmax;
-    for (int it = 0; it < 64; ++it) {
-        double mid = 0.5 * (lo + hi);
-        if (exp_disc_cdf(mid) < target) lo = mid; else hi = mid;
+    if (!(target > 0.0)) return 0.0;
+    const double* tab = inverse_table();
+    const double x = target * kInvTable;
+    int k = (int)x;
+    if (k >= kInvTable) k = kInvTable - 1;
+    const double f = x - k;
+    double q = tab[k] + f * (tab[k + 1] - tab[k]);
+    if (!(q > 0.0)) q = std::sqrt(2.0 * target);  // cdf(q) ~ q^2 / 2 near 0
+    if (q > qmax) q = qmax;
+    for (int it = 0; it < 40; ++it) {
+        const double e = std::exp(-q);
+        const double r = 1.0 - (1.0 + q) * e - target;
+        const double slope = q * e;
+        if (!(slope > 0.0)) break;
+        double next = q - r / slope;
+        if (!(next > 0.0)) next = 0.5 * q;
+        if (next > qmax) next = 0.5 * (q + qmax);
+        const double step = next - q;
+        q = next;
+        if (std::fabs(step) <= 1e-15 * q) break;
     }
-    return 0.5 * (lo + hi);
+    return q;
 }
 
 bool store(GbhEvolveState* o, double x, double y, double m, double vx, double vy) {

@@ -59,7 +59,7 @@ class Server:
 
     def close(self):
         if getattr(self, "_h", None):
-            L.lib().gbh_server_destroy(self._h)
+            L.lib().gbh_server_destroy(self._h)  # (stops the worker thread first)
             self._h = None
 
     def __del__(self):
@@ -83,6 +83,23 @@ class Server:
         return [Completion(c.id, c.submit_time, c.admit_time, c.finish_time, c.n, c.steps, c.status)
                 for c in buf[: min(k.value, cap)]]
 
+    def start(self):
+        """Continuous serving: the library's worker thread runs rounds from now on."""
+        _check(L.lib().gbh_server_start(self._h))
+
+    def stop(self):
+        """Stop the worker thread after its current round (raises if it failed)."""
+        _check(L.lib().gbh_server_stop(self._h))
+
+    def poll(self, wait: float = 0.0, cap: int = 4096):
+        """Completions the worker produced since the last poll; waits up to `wait`
+        seconds for one when none is ready."""
+        buf = (L.Completion * cap)()
+        k = C.c_uint32()
+        _check(L.lib().gbh_server_poll(self._h, buf, cap, C.byref(k), float(wait)))
+        return [Completion(c.id, c.submit_time, c.admit_time, c.finish_time, c.n, c.steps, c.status)
+                for c in buf[: k.value]]
+
     def stats(self):
         q, i, r = C.c_uint32(), C.c_uint32(), C.c_uint64()
         _check(L.lib().gbh_server_stats(self._h, C.byref(q), C.byref(i), C.byref(r)))
@@ -96,8 +113,13 @@ class Server:
                 return out
             out.extend(self.round())
 
-    def take(self, rid: int):
+    def take(self, rid: int, keep: bool = True):
+        """Final (state, accel) of a completed request; keep=False releases it
+        without copying."""
         n = self._n.pop(rid)
+        if not keep:
+            _check(L.lib().gbh_server_take_result(self._h, rid, None, None))
+            return None
         s = np.zeros((n, 8), np.float32)
         a = np.zeros((n, 4), np.float32)
         _check(L.lib().gbh_server_take_result(self._h, rid, _p(s), _p(a)))

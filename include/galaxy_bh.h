@@ -270,6 +270,9 @@ int gbh_batch_step(gbh_batch* b, uint32_t steps);
 int gbh_batch_read(gbh_batch* b, uint32_t sim, GbhEvolveState* out, GbhGpuAccel* accel);
 int gbh_batch_force_evidence(gbh_batch* b, uint32_t sim, uint32_t* stats, uint64_t* hash);
 uint64_t gbh_batch_device_bytes(gbh_batch* b);
+/* Cells and traversal nodes (internal cells + bodies) of a simulation's current
+ * tree (the last force state). */
+int gbh_batch_tree_counts(gbh_batch* b, uint32_t sim, uint32_t* cells, uint32_t* nodes);
 /* Profiling: runs ONE MORE KDK step of every simulation, un-captured and with
  * each phase's launches in sequence, and writes its kernel time per phase (ms,
  * HIP events): kick/drift + bounds, keys, sort, topology, top aggregates,
@@ -294,29 +297,45 @@ typedef struct {
     double finish_time;
     uint32_t n;
     uint32_t steps;
-    int32_t status;       /* 0 = ok, nonzero = failed (text in gbh_server_error) */
+    int32_t status;       /* 0 = ok, nonzero = failed (gbh_server_take_result gives the text) */
 } gbh_completion;
 
 /* Device bytes one resident simulation of n bodies holds (state + tree buffers);
  * the unit of HBM-budget admission. */
 uint64_t gbh_sim_device_bytes(uint32_t n);
-/* The server keeps the device arenas of retired requests for reuse by later ones
- * (up to hbm_budget_bytes of them, on top of the resident requests); they are
- * released when an allocation fails and at gbh_server_destroy. */
+/* The server keeps the device arenas of retired requests for reuse by later ones;
+ * pooled and resident arenas together stay within hbm_budget_bytes.  The pool is
+ * released when an allocation fails and at gbh_server_destroy (which first stops
+ * the worker thread, if one runs). */
 int gbh_server_create(gbh_device* dev, const gbh_server_options* opt, gbh_server** out);
 void gbh_server_destroy(gbh_server* s);
 /* Server clock in seconds (monotonic, starts at create). */
 double gbh_server_now(gbh_server* s);
 /* Queue a request; the initial state is copied.  Refused immediately (nonzero)
- * if the request alone exceeds the HBM budget. */
+ * if the request alone exceeds the HBM budget.  Safe to call from any thread,
+ * also while the worker thread runs. */
 int gbh_server_submit(gbh_server* s, const GbhEvolveState* states, uint32_t n,
                       const gbh_sim_params* params, uint32_t steps, uint64_t* id);
-/* One serving round: admit queued requests while budget and slots allow, advance
- * every in-flight request, retire finished ones.  Returns completions. */
+/* One serving round on the calling thread: admit queued requests while budget
+ * and slots allow, advance every in-flight request, retire finished ones.
+ * Returns the round's completions (*ndone of them; at most cap are written).
+ * Refused while the worker thread runs. */
 int gbh_server_round(gbh_server* s, gbh_completion* done, uint32_t cap, uint32_t* ndone);
+/* Continuous serving: a worker thread of the library runs rounds whenever
+ * requests are queued or in flight and sleeps otherwise, so submissions never
+ * wait for the caller to drive a round.  gbh_server_stop lets the current round
+ * finish, joins the thread and reports a failure of the worker, if any (a device
+ * error fails the requests concerned; a failed HIP call ends the worker). */
+int gbh_server_start(gbh_server* s);
+int gbh_server_stop(gbh_server* s);
+/* Completions the worker thread produced since the last poll (at most cap; the
+ * rest stay for the next poll).  With none ready, waits up to wait_seconds for
+ * one while the worker runs. */
+int gbh_server_poll(gbh_server* s, gbh_completion* out, uint32_t cap, uint32_t* ndone,
+                    double wait_seconds);
 /* Final state of a completed request (then released). */
 int gbh_server_take_result(gbh_server* s, uint64_t id, GbhEvolveState* out, GbhGpuAccel* accel);
-/* queued, in-flight, resident bytes */
+/* queued requests; in-flight requests and their resident bytes as of the last round */
 int gbh_server_stats(gbh_server* s, uint32_t* queued, uint32_t* inflight, uint64_t* resident);
 
 /* FP64 diagnostics over a resident state (no reference counterpart; the

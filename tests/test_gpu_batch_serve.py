@@ -121,6 +121,52 @@ def test_server_results_same_alone_and_in_mix(oracle, gpu, streams):
     assert np.array_equal(s.view(np.uint32), mixed[ids[1]][0].view(np.uint32))
 
 
+def test_server_worker_thread_continuous_serving(oracle, gpu):
+    """The library's worker thread runs the rounds while requests arrive from the
+    caller's thread: every result still equals the oracle's trajectory, rounds with
+    an unchanged in-flight set replay HIP graphs (two long requests run many rounds
+    together), and manual rounds are refused while the worker runs."""
+    import time
+
+    sims = mixed_sims()
+    steps = [60, 60, 20, 10, 40]
+    print("server worker thread: 5 requests, staggered", flush=True)
+    srv = Server(gpu, hbm_budget_bytes=8 << 30, steps_per_round=10, streams=2)
+    srv.start()
+    with pytest.raises(Exception, match="worker thread"):
+        srv.round()
+    ids, done = [], {}
+    for (s, c, dt), k in zip(sims, steps):
+        ids.append(srv.submit(s, c, dt, k))
+        for d in srv.poll(wait=0.002):
+            done[d.id] = d
+        time.sleep(0.003)
+    t_end = time.time() + 60
+    while len(done) < len(ids) and time.time() < t_end:
+        for d in srv.poll(wait=0.05):
+            done[d.id] = d
+    srv.stop()
+    assert sorted(done) == sorted(ids) and all(d.status == 0 for d in done.values())
+    assert all(d.admit_time >= d.submit_time and d.finish_time >= d.admit_time for d in done.values())
+    for rid, (s0, cfg, dt), k in zip(ids, sims, steps):
+        rs, ra = oracle.f32_evolve(s0, k, dt, ocfg(oracle, cfg))
+        s, a = srv.take(rid)
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), rid
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), rid
+    # the worker can be restarted and drives later submissions
+    rid = srv.submit(*sims[2], 10)
+    srv.start()
+    got = []
+    t_end = time.time() + 30
+    while not got and time.time() < t_end:
+        got = srv.poll(wait=0.05)
+    srv.stop()
+    assert [d.id for d in got] == [rid]
+    rs, _ = oracle.f32_evolve(sims[2][0], 10, sims[2][2], ocfg(oracle, sims[2][1]))
+    assert np.array_equal(srv.take(rid)[0].view(np.uint32), rs.view(np.uint32))
+    srv.close()
+
+
 def test_server_budget_admission_and_staggered_arrivals(oracle, gpu):
     sims = mixed_sims()
     one = Server.request_bytes(5000)
