@@ -10,7 +10,9 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgalaxy_bh.so")
+# GALAXY_BH_LIB selects another build of the library (the host-sanitized one of
+# galaxy_amd/build.py --sanitize, for CPU test runs).
+LIB_PATH = os.environ.get("GALAXY_BH_LIB") or os.path.join(HERE, "libgalaxy_bh.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "galaxy_bh.h")
 
 _lib = None

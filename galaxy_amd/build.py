@@ -34,19 +34,35 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+# Host-side sanitizers for CPU test runs (no GPU): AddressSanitizer and
+# UndefinedBehaviorSanitizer on the host pass only; the device code is built as
+# usual.  The result goes to galaxy_amd/_san/ and is loaded when GALAXY_BH_LIB
+# points there (with the clang ASan runtime preloaded).
+SANITIZE_FLAGS = [
+    "-Xarch_host", "-fsanitize=address",
+    "-Xarch_host", "-fsanitize=undefined",
+    "-Xarch_host", "-shared-libsan",
+    "-Xarch_host", "-fno-omit-frame-pointer",
+    "-g",
+]
+SAN_OUT = os.path.join(HERE, "_san", "libgalaxy_bh.so")
+
+
+def build(verbose: bool = False, sanitize: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    out = SAN_OUT if sanitize else OUT
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *FLAGS, *(SANITIZE_FLAGS if sanitize else []), "-o", tmp, *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libgalaxy_bh.so")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, sanitize="--sanitize" in sys.argv[1:]))

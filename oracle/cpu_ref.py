@@ -20,7 +20,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libcpu_ref.so")
+# GALAXY_CPU_REF_LIB selects another build of the oracle (the sanitized one of
+# `make -C oracle sanitize`).
+_LIB_PATH = os.environ.get("GALAXY_CPU_REF_LIB") or os.path.join(_HERE, "libcpu_ref.so")
 _lib = None
 
 
