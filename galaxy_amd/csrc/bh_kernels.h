@@ -1546,7 +1546,12 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
 constexpr uint32_t kTopThreads = 1024;
 constexpr uint32_t kTopLds = 4096;  // aggregate slots: spanning cells, then preloaded children
 
-__global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims) {
+// Also gathers the sticky error words as k_collect_errors does (errs[0] = the
+// set's, errs[1 + s] = simulation s's): it runs last in a build, and no kernel
+// after it in a step sets them.
+__global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims,
+                                                         const uint32_t* __restrict__ ctl,
+                                                         uint32_t* __restrict__ errs) {
     // In LDS mode the aggregates live in s_mxy (m, x, y planes); otherwise the
     // same memory holds the spanning cells bucketed by depth (kTopListCap slots).
     __shared__ double s_mxy[3 * kTopLds];
@@ -1560,6 +1565,10 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     uint32_t* s_list = (uint32_t*)s_mxy;
     constexpr uint32_t kTopListCap = 3u * kTopLds * 2u;
     const SimDesc& S = sim_at(sims, blockIdx.x);
+    if (threadIdx.x == 0) {
+        errs[1 + blockIdx.x] = S.meta->err;
+        if (blockIdx.x == 0) errs[0] = ctl[kCtlError];
+    }
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
     const bool in_lds = nspan <= kTopLds / 5;

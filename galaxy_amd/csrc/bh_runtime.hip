@@ -280,6 +280,7 @@ struct SimSet {
         ensure(d_tflags, cap_tflags, ttopo.size());
         ensure(d_tsum, cap_tsum, kTopoSum * ttopo.size());
         ensure(d_psum, cap_psum, kTopoSum * ttopo.size());
+        prepare_errors();
         HIP_OK(hipMemcpyAsync(d_desc, desc.data(), sizeof(SimDesc) * desc.size(),
                               hipMemcpyHostToDevice, stream));
         auto put = [&](TileRef* dst, const std::vector<TileRef>& v) {
@@ -357,7 +358,8 @@ struct SimSet {
         HIP_OK(hipGetLastError());
     }
     void agg(hipStream_t s) {
-        hipLaunchKernelGGL(k_agg_top, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s, d_desc);
+        hipLaunchKernelGGL(k_agg_top, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s, d_desc, d_ctl,
+                           d_errs);
         HIP_OK(hipGetLastError());
     }
     void build(hipStream_t s) {
@@ -399,7 +401,9 @@ struct SimSet {
     // One gather kernel and one copy, whatever the number of simulations.
     // enqueue_errors puts the gather and its copy into pinned host memory on the
     // stream; take_errors reads the words once the stream has been synchronized.
-    void enqueue_errors(hipStream_t s) {
+    // The error gather buffers (sized by upload, so nothing is allocated while a
+    // caller captures the stream).
+    void prepare_errors() {
         const uint32_t nsims = (uint32_t)desc.size();
         ensure(d_errs, cap_errs, 1ull + nsims);
         if (cap_herrs < 1ull + nsims) {
@@ -409,10 +413,19 @@ struct SimSet {
             HIP_OK(hipHostMalloc((void**)&h_errs, 4ull * (1ull + nsims), hipHostMallocDefault));
             cap_herrs = 1ull + nsims;
         }
+    }
+    void enqueue_errors(hipStream_t s) {
+        const uint32_t nsims = (uint32_t)desc.size();
+        prepare_errors();
         hipLaunchKernelGGL(k_collect_errors, dim3(div_up(nsims + 1u, 256u)), dim3(256), 0, s, d_desc, nsims,
                            d_ctl, d_errs);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(h_errs, d_errs, 4ull * (1ull + nsims), hipMemcpyDeviceToHost, s));
+        copy_errors(s);
+    }
+    // Copy of the gathered words to pinned host memory: after a build, k_agg_top
+    // has gathered them (no kernel after it in a build sets error words).
+    void copy_errors(hipStream_t s) {
+        HIP_OK(hipMemcpyAsync(h_errs, d_errs, 4ull * (1ull + desc.size()), hipMemcpyDeviceToHost, s));
     }
     uint32_t take_errors(std::vector<uint32_t>* per_sim = nullptr) const {
         const uint32_t nsims = (uint32_t)desc.size();
@@ -662,6 +675,7 @@ struct gbh_tree {
     SimSet set;
     hipEvent_t ev[8] = {};
     double last_force_ms = 0.0;
+    hipGraphExec_t rebuild_graph = nullptr;  // the build, captured at the first rebuild
 };
 
 extern "C" {
@@ -951,6 +965,7 @@ void gbh_tree_destroy(gbh_tree* t) {
     if (!t) return;
     for (auto& e : t->ev)
         if (e) (void)hipEventDestroy(e);
+    if (t->rebuild_graph) (void)hipGraphExecDestroy(t->rebuild_graph);
     if (t->arena) (void)hipFree(t->arena);
     delete t;
 }
@@ -981,35 +996,35 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
         double t0 = now_seconds();
         // a failed rebuild leaves no usable tree until the next one succeeds
         t->built = false;
-        HIP_OK(hipEventRecord(t->ev[0], s));
-        t->set.bounds(0, s, true);  // also clears the error words of the previous rebuild
+        // The whole build is one HIP graph, captured at the first rebuild: bounds
+        // (which also clears the previous rebuild's error words and the traversal
+        // queue), keys, the radix passes, topology, top aggregates with the error
+        // gather, and the copy of the error words to pinned host memory.  The
+        // kernels read the configuration from the simulation table, so a changed
+        // configuration needs no new capture.
+        if (!t->rebuild_graph) {
+            hipGraph_t gr = nullptr;
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            t->set.bounds(0, s, true);
+            t->set.build(s);
+            t->set.copy_errors(s);
+            HIP_OK(hipStreamEndCapture(s, &gr));
+            const hipError_t e = hipGraphInstantiate(&t->rebuild_graph, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            HIP_OK(e);
+        }
+        HIP_OK(hipGraphLaunch(t->rebuild_graph, s));
         t->queue_zero = true;
-        HIP_OK(hipEventRecord(t->ev[1], s));
-        t->set.keys(s);
-        HIP_OK(hipEventRecord(t->ev[2], s));
-        t->set.sort(s);
-        HIP_OK(hipEventRecord(t->ev[3], s));
-        t->set.topo(s);
-        HIP_OK(hipEventRecord(t->ev[4], s));
-        t->set.agg(s);
-        HIP_OK(hipEventRecord(t->ev[5], s));
-        t->set.enqueue_errors(s);
         HIP_OK(hipStreamSynchronize(s));
-        double wall = now_seconds() - t0;
         uint32_t err = t->set.take_errors();
         if (err) {
             set_error("BH #2D rebuild failed on the device: " + error_text(err));
             return -1;
         }
         t->built = true;
-        if (timing) {
-            double b = ms_between(t->ev[0], t->ev[1]) * 1e-3, m = ms_between(t->ev[1], t->ev[2]) * 1e-3;
-            double r = ms_between(t->ev[2], t->ev[3]) * 1e-3, tp = ms_between(t->ev[3], t->ev[4]) * 1e-3;
-            double a = ms_between(t->ev[4], t->ev[5]) * 1e-3;
-            double gpu = b + m + r + tp + a;
-            double host = wall > gpu ? wall - gpu : 0.0;  // launch + synchronization
-            *timing = gbh_build_timing{b + host, m, r, 0.0, tp, a};
-        }
+        // the stages run as one graph: the call's wall time is reported as the
+        // bounds stage, the slot of the first stage
+        if (timing) *timing = gbh_build_timing{now_seconds() - t0, 0.0, 0.0, 0.0, 0.0, 0.0};
         return 0;
     });
 }

@@ -133,7 +133,9 @@ int gbh_tree_create(gbh_runtime* rt, gbh_state* st, gbh_tree** out);
  * compact cell layout; same kernels and results contract. */
 int gbh_tree_create_large(gbh_runtime* rt, gbh_state* st, gbh_tree** out);
 void gbh_tree_destroy(gbh_tree* tree);
-/* rebuild (nbody_parallel.rs:555-826) */
+/* rebuild (nbody_parallel.rs:555-826).  The stages run as one HIP graph (captured
+ * at the tree's first rebuild), so the call's wall time is reported in
+ * timing->bounds_seconds and the other stage slots are 0. */
 int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* tree, const gbh_config* cfg,
                      gbh_build_timing* timing);
 /* force (nbody_parallel.rs:828-897): writes the state's acceleration buffer. */

@@ -135,6 +135,23 @@ def test_force_theta_differs_from_rebuild(oracle, gpu, rt):
     check_force(oracle, gpu, rt, ev, tree, s, force_cfg, (st, bodies, entries, cells))
 
 
+def test_rebuild_graph_follows_config_changes(oracle, gpu, rt):
+    """The rebuild replays one captured graph per tree; the configuration is read
+    from the simulation table, so rebuilding the same tree with another leaf size,
+    depth cap and theta gives that configuration's tree and forces."""
+    s = ics.rotating_disc(6000, 31, softening=0.01)
+    ev = gpu.create_evolving_state(s)
+    tree = rt.create_tree(gpu, ev)
+    for cfg in [Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16),
+                Config(theta=0.7, softening_kpc=0.02, g=1.0, bucket=1, max_depth=16),
+                Config(theta=0.3, softening_kpc=0.01, g=2.0, bucket=16, max_depth=5),
+                Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)]:
+        print(f"graph rebuild: bucket={cfg.bucket} theta={cfg.theta} max_depth={cfg.max_depth}", flush=True)
+        rt.rebuild(gpu, ev, tree, cfg)
+        packed = check_tree(oracle, gpu, rt, tree, s, cfg)
+        check_force(oracle, gpu, rt, ev, tree, s, cfg, packed)
+
+
 def test_repeat_rebuild_is_identical(oracle, gpu, rt):
     s = ics.uniform_collapse(20000, 44)
     cfg = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)
