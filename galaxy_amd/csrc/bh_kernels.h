@@ -28,7 +28,6 @@ constexpr uint32_t kWaveTargets = 64;    // traversal targets per wavefront
 constexpr uint32_t kLevels = 17;         // Morton depths 0..16
 constexpr uint32_t kInfoLeaf = 1u << 8;
 constexpr uint32_t kInfoSpanShift = 9;   // (span slot + 1) << 9 for internal cells that cross tiles
-constexpr uint32_t kSpanSlotsLds = 4096; // spanning cells aggregated in LDS
 
 // Control words shared by the kernels of one simulation set (tile tickets are
 // per simulation, SimMeta::ticket): the sticky error bits, the traversal's work
@@ -117,6 +116,8 @@ struct SimMeta {
     uint32_t nspan;
     uint32_t err;
     uint32_t theta_bits;    // f32 theta the thresholds were computed for
+    uint32_t pow4;          // T[d] = T[0] * 4^-d exactly for d = 1..15 (see pow4_thresholds)
+    uint32_t t0_bits;       // bit pattern of T[0]
     uint32_t hist[4 * 256]; // per-pass digit histograms of the Morton keys
     // tile tickets of the simulation: radix passes 0..3, topology (4)
     uint32_t ticket[8];
@@ -151,6 +152,7 @@ struct __attribute__((aligned(16))) SimDesc {
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
     uint32_t* span;    // spanning internal cells (span_cap entries)
+    uint32_t* spanbuf; // the same bucketed by depth (k_agg_top, listed mode)
     NodeHot* nodes;    // traversal node stream (cell_cap + n entries)
     NodeCold* ncold;   // its cold records
     uint32_t* nbase;   // per sorted position: node index of the column's first cell
@@ -386,6 +388,26 @@ __device__ float accept_threshold(float half, float theta) {
         if (pass(mid)) hi = mid; else lo = mid;
     }
     return __uint_as_float(hi);
+}
+
+// Whether a simulation's opening thresholds scale by powers of four: T[0] is a
+// finite normal f32 and T[d] = T[0] * 4^-d exactly (both ways) for d = 1..15.
+// The thresholds are defined by w_d / fl(sqrt(d2)) < theta with w_d = w_0 2^-d,
+// and scaling d2 by 4^d scales fl(sqrt(d2)) by 2^d exactly while all values stay
+// normal, so this holds except near the ends of the f32 range.  Then, for every
+// finite d2 >= 0, d2 >= T[d] <=> bits(d2) + d * 2^24 >= bits(T[0]) as unsigned
+// integers: for normal d2 the sum is bits(d2 4^d) (or exceeds bits(FLT_MAX) when
+// d2 4^d overflows, and then d2 > T[d] since T[0] <= FLT_MAX); for subnormal d2
+// both sides are false since T[0] >= 2^-126 4^d.  Called by a whole wavefront
+// whose lane d < kCodes holds T[d]; returns 1 when the property holds.
+__device__ __forceinline__ uint32_t pow4_thresholds(float Td, float T0) {
+    const int d = (int)lane_id();
+    bool ok = true;
+    if (d < (int)kCodes) {
+        const bool normal = T0 >= 1.17549435e-38f && T0 < __builtin_inff();
+        ok = normal && (d == 0 || (Td == __builtin_ldexpf(T0, -2 * d) && __builtin_ldexpf(Td, 2 * d) == T0));
+    }
+    return __ballot(!ok) == 0ull ? 1u : 0u;
 }
 
 // Correctly rounded f32 square root and reciprocal from the f64 hardware
@@ -715,13 +737,20 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
             // centres of mass lie in the root box), so d2 stays finite
             M->geom_ok = half < 1.0e18 ? 1u : 0u;
         }
+        float T = 0.0f;
         if (threadIdx.x < kLevels) {
             // half at depth d is half * 2^-d exactly in f64; one rounding to f32
             double hd = half;
             for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-            const float T = accept_threshold((float)hd, S.theta);
+            T = accept_threshold((float)hd, S.theta);
             M->T[threadIdx.x] = T;
             if (threadIdx.x < kCodes) M->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
+        }
+        const float T0 = __shfl(T, 0);
+        const uint32_t p4 = pow4_thresholds(T, T0);
+        if (threadIdx.x == 0) {
+            M->pow4 = p4;
+            M->t0_bits = __float_as_uint(T0);
         }
     }
     // Morton keys of this tile
@@ -1536,15 +1565,48 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
 
 // ----------------------------------------------------------------------------
 // K6: aggregates of the spanning internal cells (the top of the tree), one
-// workgroup per simulation, bottom-up by depth.  Every global read happens
-// before the depth rounds: each spanning cell's children become references to
-// either another spanning cell's running value or a preloaded aggregate of a
-// non-spanning child, all in LDS, so a round is an LDS round trip.  Lists too
-// long for that (more than kTopLds / 5 spanning cells) are aggregated through
-// global memory in the same order.
+// workgroup per simulation, bottom-up by depth.  The footprint is small (256
+// threads, about 37 KB of LDS) so that these workgroups run beside the other
+// group's persistent traversal instead of waiting for it to end.
+//  - Up to kTopLdsCells spanning cells ("LDS mode"): every global read happens
+//    before the depth rounds; each spanning cell's children become references
+//    to either another spanning cell's running value or a preloaded aggregate
+//    of a non-spanning child, all in LDS, so a round is an LDS round trip.
+//  - More ("listed mode"): the cells are bucketed by depth into the
+//    simulation's spanbuf (a counting sort), and each depth round aggregates its
+//    own bucket through global memory, children in quadrant order as ever.
 // ----------------------------------------------------------------------------
-constexpr uint32_t kTopThreads = 1024;
-constexpr uint32_t kTopLds = 4096;  // aggregate slots: spanning cells, then preloaded children
+constexpr uint32_t kTopThreads = 256;
+constexpr uint32_t kTopLdsCells = 256;            // spanning cells aggregated in LDS
+constexpr uint32_t kTopLds = 5 * kTopLdsCells;    // their slots, then preloaded children
+
+// Aggregate of spanning cell idx (depth d) from its children in global memory;
+// writes its f64 aggregate, its cell's f32 aggregates and its node's.
+__device__ __forceinline__ Agg agg_span_global(const SimDesc& S, uint32_t idx) {
+    const uint4 ch = S.cchild[idx];
+    double mass = 0.0, wx = 0.0, wy = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t c = (&ch.x)[q];
+        if (c == kNoChild) continue;
+        const Agg a = S.cagg[c];
+        mass = mass + a.m;
+        wx = wx + a.m * a.x;
+        wy = wy + a.m * a.y;
+    }
+    return Agg{mass, wx / mass, wy / mass, 0.0};
+}
+
+__device__ __forceinline__ void store_span(const SimDesc& S, uint32_t idx, uint32_t node, const Agg& a) {
+    S.cagg[idx] = a;
+    S.cells[idx].comx = (float)a.x;
+    S.cells[idx].comy = (float)a.y;
+    S.cells[idx].mass = (float)a.m;
+    NodeHot* nd = &S.nodes[node];
+    nd->x = (float)a.x;
+    nd->y = (float)a.y;
+    nd->m = (float)a.m;
+}
 
 // Also gathers the sticky error words as k_collect_errors does (errs[0] = the
 // set's, errs[1 + s] = simulation s's): it runs last in a build, and no kernel
@@ -1552,18 +1614,11 @@ constexpr uint32_t kTopLds = 4096;  // aggregate slots: spanning cells, then pre
 __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims,
                                                          const uint32_t* __restrict__ ctl,
                                                          uint32_t* __restrict__ errs) {
-    // In LDS mode the aggregates live in s_mxy (m, x, y planes); otherwise the
-    // same memory holds the spanning cells bucketed by depth (kTopListCap slots).
-    __shared__ double s_mxy[3 * kTopLds];
-    __shared__ uint32_t s_ref[kTopLds / 5][4];
-    __shared__ uint32_t s_idx[kTopLds / 5], s_node[kTopLds / 5];  // cell and node index
-    __shared__ uint8_t s_depth[kTopLds / 5];
+    __shared__ double s_m[kTopLds], s_x[kTopLds], s_y[kTopLds];
+    __shared__ uint32_t s_ref[kTopLdsCells][4];
+    __shared__ uint32_t s_idx[kTopLdsCells], s_node[kTopLdsCells];  // cell and node index
+    __shared__ uint8_t s_depth[kTopLdsCells];
     __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
-    double* s_m = s_mxy;
-    double* s_x = s_mxy + kTopLds;
-    double* s_y = s_mxy + 2 * kTopLds;
-    uint32_t* s_list = (uint32_t*)s_mxy;
-    constexpr uint32_t kTopListCap = 3u * kTopLds * 2u;
     const SimDesc& S = sim_at(sims, blockIdx.x);
     if (threadIdx.x == 0) {
         errs[1 + blockIdx.x] = S.meta->err;
@@ -1571,12 +1626,9 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     }
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
-    const bool in_lds = nspan <= kTopLds / 5;
-    const bool listed = !in_lds && nspan <= kTopListCap;
-    if (threadIdx.x == 0) s_next = nspan;
-    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
-    __syncthreads();
-    if (in_lds) {
+    if (nspan <= kTopLdsCells) {
+        if (threadIdx.x == 0) s_next = nspan;
+        __syncthreads();
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
             const uint32_t idx = S.span[i];
             const uint32_t depth = S.cells[idx].info & 255u;
@@ -1610,57 +1662,10 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             }
         }
         __syncthreads();
-    } else if (listed) {
-        // counting sort of the spanning cells by depth, so that every depth
-        // round below visits only its own cells
-        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
-            atomicAdd(&s_cnt[min(S.cells[S.span[i]].info & 255u, kLevels - 1u)], 1u);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (uint32_t d = 0; d < kLevels; ++d) {
-                s_off[d] = acc;
-                acc += s_cnt[d];
-                s_cnt[d] = 0u;
-            }
-            s_off[kLevels] = acc;
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-            const uint32_t idx = S.span[i];
-            const uint32_t d = min(S.cells[idx].info & 255u, kLevels - 1u);
-            s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = idx;
-        }
-        __syncthreads();
-    }
-    for (int d = 15; d >= 0; --d) {
-        const uint32_t lo = listed ? s_off[d] : 0u, hi = listed ? s_off[d + 1] : nspan;
-        for (uint32_t k = lo + threadIdx.x; k < hi; k += kTopThreads) {
-            // the cell (index i of the span list, cell idx) and its depth
-            const uint32_t i = k;
-            uint32_t idx, depth;
-            if (in_lds) {
-                depth = s_depth[i];
-                idx = s_idx[i];
-            } else if (listed) {  // bucket d holds the cells of depth d
-                idx = s_list[k];
-                depth = (uint32_t)d;
-            } else {
-                idx = S.span[i];
-                depth = S.cells[idx].info & 255u;
-            }
-            if (depth != (uint32_t)d) continue;
-            // node index first: its loads issued after this cell's stores would
-            // wait for them
-            uint32_t node;
-            if (in_lds) {
-                node = s_node[i];
-            } else {
-                const uint32_t st = S.cells[idx].start;
-                node = S.nbase[st] + (depth - S.cdmin[st]);
-            }
-            double mass = 0.0, wx = 0.0, wy = 0.0;
-            if (in_lds) {
+        for (int d = 15; d >= 0; --d) {
+            for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+                if (s_depth[i] != (uint32_t)d) continue;
+                double mass = 0.0, wx = 0.0, wy = 0.0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t r = s_ref[i][q];
@@ -1670,37 +1675,47 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
                     wx = wx + cm * s_x[r];
                     wy = wy + cm * s_y[r];
                 }
-            } else {
-                const uint4 ch = S.cchild[idx];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t c = (&ch.x)[q];
-                    if (c == kNoChild) continue;
-                    const Agg a = S.cagg[c];
-                    mass = mass + a.m;
-                    wx = wx + a.m * a.x;
-                    wy = wy + a.m * a.y;
-                }
+                const Agg a{mass, wx / mass, wy / mass, 0.0};
+                s_m[i] = a.m;
+                s_x[i] = a.x;
+                s_y[i] = a.y;
+                store_span(S, s_idx[i], s_node[i], a);
             }
-            const double comx = wx / mass, comy = wy / mass;
-            if (in_lds) {
-                s_m[i] = mass;
-                s_x[i] = comx;
-                s_y[i] = comy;
-            }
-            S.cagg[idx] = Agg{mass, comx, comy, 0.0};
-            S.cells[idx].comx = (float)comx;
-            S.cells[idx].comy = (float)comy;
-            S.cells[idx].mass = (float)mass;
-            NodeHot* nd = &S.nodes[node];
-            nd->x = (float)comx;
-            nd->y = (float)comy;
-            nd->m = (float)mass;
+            // the next depth reads only LDS: no wait for this depth's global stores
+            lds_barrier();
         }
-        // In LDS mode the next depth reads only LDS, so the barrier need not
-        // wait for this depth's global stores; otherwise it reads them.
-        if (in_lds) lds_barrier();
-        else __syncthreads();
+        return;
+    }
+    // listed mode: counting sort of the spanning cells by depth into spanbuf
+    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
+        atomicAdd(&s_cnt[min(S.cells[S.span[i]].info & 255u, kLevels - 1u)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t d = 0; d < kLevels; ++d) {
+            s_off[d] = acc;
+            acc += s_cnt[d];
+            s_cnt[d] = 0u;
+        }
+        s_off[kLevels] = acc;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+        const uint32_t idx = S.span[i];
+        const uint32_t d = min(S.cells[idx].info & 255u, kLevels - 1u);
+        S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = idx;
+    }
+    __syncthreads();  // (also orders the list's global stores before the rounds' loads)
+    for (int d = 15; d >= 0; --d) {
+        for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kTopThreads) {
+            const uint32_t idx = S.spanbuf[k];
+            const uint32_t st = S.cells[idx].start;
+            const uint32_t node = S.nbase[st] + ((uint32_t)d - S.cdmin[st]);
+            store_span(S, idx, node, agg_span_global(S, idx));
+        }
+        __syncthreads();  // the next depth reads this depth's global stores
     }
 }
 
@@ -1709,14 +1724,21 @@ __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
     const uint32_t s = blockIdx.x;
     if (s >= nsims) return;
     const SimDesc& S = sim_at(sims, s);
+    float T = 0.0f;
     if (threadIdx.x < kLevels) {
         double hd = S.meta->root[2];
         for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-        const float T = accept_threshold((float)hd, S.theta);
+        T = accept_threshold((float)hd, S.theta);
         S.meta->T[threadIdx.x] = T;
         if (threadIdx.x < kCodes) S.meta->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
     }
-    if (threadIdx.x == 0) S.meta->theta_bits = __float_as_uint(S.theta);
+    const float T0 = __shfl(T, 0);
+    const uint32_t p4 = pow4_thresholds(T, T0);  // (one wavefront: blockDim.x == 64)
+    if (threadIdx.x == 0) {
+        S.meta->theta_bits = __float_as_uint(S.theta);
+        S.meta->pow4 = p4;
+        S.meta->t0_bits = __float_as_uint(T0);
+    }
 }
 
 // Thresholds of a device-built tree for a new theta: every cell, and the cold
@@ -1785,16 +1807,39 @@ struct WalkEvidence {
 // 0 < eps2 <= FLT_MAX), where point_mass_pos equals point_mass_d bit for bit.
 // The walk is software-pipelined: the next node's gather is issued before the
 // interaction of the current one is evaluated.  Whatever the node words hold,
-// every step advances the cursor by at least one node and the early gather is
-// clamped to index nn (the node arrays hold at least nn + 1 records, and the
-// value read there is never used), so a walk always ends and stays in bounds.
-// `tab` is the byte address of the wavefront's code table in LDS.
-template <bool kEvidence, bool kGeneral, bool kFast>
+// every step advances the cursor by at least one node, so a walk always ends.
+// The hot records are read through a buffer resource whose size is the node
+// array's: a gather beyond it (the early gather past the last node, whose value
+// is never used) returns zeros instead of touching memory, with no clamp or
+// 64-bit address arithmetic in the loop.  `tab` is the byte address of the
+// wavefront's code table in LDS.
+// Buffer resource over `records` 16-byte node records (raw buffer, 32-bit
+// offsets; loads beyond the size return zeros).  gfx9 raw-buffer word 3.
+constexpr int kRawBufferWord3 = 0x00020000;
+// The arguments must be wave-uniform (they are made explicitly so, which keeps
+// the resource in scalar registers).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t node_buffer(const void* p, uint32_t records) {
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    const uint32_t bytes = __builtin_amdgcn_readfirstlane(records * 16u);
+    void* base = (void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, kRawBufferWord3);
+}
+__device__ __forceinline__ F4v load_node(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+    return __builtin_bit_cast(F4v, __builtin_amdgcn_raw_buffer_load_b128(r, index * 16u, 0, 0));
+}
+
+// kPow4: the thresholds scale by powers of four (SimMeta::pow4), so the opening
+// test is an integer comparison against T[0] (pow4_thresholds) instead of a
+// code-table read from LDS; code 0 (bodies, the root) has threshold 0.
+template <bool kEvidence, bool kGeneral, bool kFast, bool kPow4 = false>
 struct NodeWalk {
-    const __attribute__((address_space(1))) F4v* nh;
+    __amdgpu_buffer_rsrc_t nh;
     const __attribute__((address_space(1))) U4v* nc;
     uint32_t tab, nn, p, self;
     float tx, ty, eps2, G;
+    uint32_t t0_bits = 0;
     float ax = 0.f, ay = 0.f;
     WalkEvidence ev;
 
@@ -1809,19 +1854,23 @@ struct NodeWalk {
         const float d2 = dd + ee;
         const uint32_t size_m1 = w >> kSizeShift;  // subtree [c, c + size_m1]
         bool inside;
-        float T;
+        bool open_ok;  // d2 >= the node's threshold
         U4v l;
         if (kGeneral || kEvidence) l = nc[c];  // start, count, T, tag
         if (kGeneral) {
             inside = (p - l.x) < l.y;
-            T = __uint_as_float(l.z);
+            open_ok = d2 >= __uint_as_float(l.z);
+        } else if (kPow4) {
+            inside = (self - c) <= size_m1;
+            const uint32_t code4 = w & kCodeField;  // code << 2
+            open_ok = code4 == 0u || __float_as_uint(d2) + (code4 << 22) >= t0_bits;
         } else {
             inside = (self - c) <= size_m1;
-            T = *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
+            open_ok = d2 >= *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
         }
-        const bool acc = !inside && d2 >= T;
+        const bool acc = !inside && open_ok;
         const uint32_t next = c + 1u + (acc ? size_m1 : 0u);  // at least one node per step
-        gn = nh[min(next, nn)];
+        gn = load_node(nh, next);
         if (kEvidence) {
             if (!(l.w & kTagBody) || (l.w & kTagFirst)) ++ev.visited;  // a cell (a leaf: its first body)
         }
@@ -1845,7 +1894,7 @@ struct NodeWalk {
 
     // Two steps per trip with the records alternating between g0 and g1.
     __device__ __forceinline__ void run(uint32_t c) {
-        F4v g0 = nh[c], g1;
+        F4v g0 = load_node(nh, c), g1;
         while (c < nn) {
             c = step(c, g0, g1);
             if (c >= nn) break;
@@ -1936,6 +1985,8 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const uint32_t nn = __builtin_amdgcn_readfirstlane(min(S.meta->nnodes, S.node_cap - 1u));
         const uint32_t general = __builtin_amdgcn_readfirstlane(S.meta->mode);
         const uint32_t geom_ok = __builtin_amdgcn_readfirstlane(S.meta->geom_ok);
+        const uint32_t pow4 = __builtin_amdgcn_readfirstlane(S.meta->pow4);
+        const uint32_t t0_bits = __builtin_amdgcn_readfirstlane(S.meta->t0_bits);
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
@@ -1954,10 +2005,15 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         float ax = 0.f, ay = 0.f;
         WalkEvidence ev;
         const uint32_t c0 = valid ? 0u : nn;
-        const GlobalF4 nh = (GlobalF4)(const void*)S.nodes;
+        // node_cap < 2^26 records, so the byte size fits the resource's 32 bits
+        const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, S.node_cap);
         const GlobalU4 nc = (GlobalU4)(const void*)S.ncold;
         if (general) {
             NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            wk.run(c0);
+            ax = wk.ax, ay = wk.ay, ev = wk.ev;
+        } else if (fast && pow4 && !kEvidence) {
+            NodeWalk<kEvidence, false, true, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, t0_bits};
             wk.run(c0);
             ax = wk.ax, ay = wk.ay, ev = wk.ev;
         } else if (fast) {

@@ -115,6 +115,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
     d.span = (uint32_t*)take(4ull * scap);
+    d.spanbuf = (uint32_t*)take(4ull * scap);
     d.nodes = (NodeHot*)take(sizeof(NodeHot) * (cap + n));
     d.ncold = (NodeCold*)take(sizeof(NodeCold) * (cap + n));
     d.nbase = (uint32_t*)take(4ull * n);
