@@ -782,12 +782,14 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
 // wavefront ballots, LDS-staged scatter, decoupled look-back across tiles).
 // Equals the stable sort_by_key of flat.rs:233-234 after four passes.
 // ----------------------------------------------------------------------------
+// One pass over the tile of workgroup `block` of `grid` (every thread of the
+// workgroup calls it).
 template <uint32_t kKPT>
-__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __restrict__ sims,
-                                                            const TileRef* __restrict__ stiles,
-                                                            uint32_t nstiles, uint32_t pass,
-                                                            uint32_t* __restrict__ ctl,
-                                                            uint32_t* __restrict__ sflags) {
+__device__ __forceinline__ void sort_pass_body(const SimDesc* __restrict__ sims,
+                                               const TileRef* __restrict__ stiles, uint32_t nstiles,
+                                               uint32_t pass, uint32_t* __restrict__ ctl,
+                                               uint32_t* __restrict__ sflags, uint32_t block,
+                                               uint32_t grid) {
     constexpr uint32_t kSortTile = kSortThreads * kKPT;
     constexpr uint32_t kWaves = kSortThreads / 64;
     constexpr uint32_t kChunk = 64 * kKPT;  // keys per wavefront
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     // in order and a look-back only waits for tiles whose workgroups run.
     // (One ticket counter per simulation: a single counter for every tile
     // serialises on one address.)
-    const uint32_t sim = stiles[xcd_tile(blockIdx.x, gridDim.x)].sim;
+    const uint32_t sim = stiles[xcd_tile(block, grid)].sim;
     const SimDesc& S = sim_at(sims, sim);
     if (threadIdx.x == 0) s_ticket = S.first_sort_tile + atomicAdd(&S.meta->ticket[pass], 1u);
     for (uint32_t k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&s_wcnt[0][0])[k] = 0u;
@@ -904,6 +906,29 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __res
     }
 }
 
+template <uint32_t kKPT>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const SimDesc* __restrict__ sims,
+                                                            const TileRef* __restrict__ stiles,
+                                                            uint32_t nstiles, uint32_t pass,
+                                                            uint32_t* __restrict__ ctl,
+                                                            uint32_t* __restrict__ sflags) {
+    sort_pass_body<kKPT>(sims, stiles, nstiles, pass, ctl, sflags, blockIdx.x, gridDim.x);
+}
+
+// All four passes of a set with a single sort tile (one simulation of at most
+// one tile's keys), in one workgroup: the passes follow one another after a
+// barrier, which also makes each pass's global stores visible to the next.
+template <uint32_t kKPT>
+__global__ __launch_bounds__(kSortThreads) void k_sort_single(const SimDesc* __restrict__ sims,
+                                                              const TileRef* __restrict__ stiles,
+                                                              uint32_t* __restrict__ ctl,
+                                                              uint32_t* __restrict__ sflags) {
+    for (uint32_t pass = 0; pass < 4; ++pass) {
+        sort_pass_body<kKPT>(sims, stiles, 1u, pass, ctl, sflags, 0u, 1u);
+        __syncthreads();
+    }
+}
+
 // ----------------------------------------------------------------------------
 // K4: cell columns.  Position p starts cells at depths dmin(p)..dleaf(p): dmin
 // from the common prefix with its predecessor, existence from the parent group's
@@ -939,12 +964,12 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 // (x, y, m) of the body at sorted position p: its reads go to random body
 // indices (inside the last radix pass their latency held every tile's
 // workgroup), here they overlap the boundary work.
-__global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
-                                                           const TileRef* __restrict__ ttiles,
-                                                           uint32_t* __restrict__ tsum) {
+__device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
+                                              const TileRef* __restrict__ ttiles,
+                                              uint32_t* __restrict__ tsum, uint32_t block, uint32_t grid) {
     // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
     __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t t = xcd_tile(block, grid);
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
@@ -999,6 +1024,12 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __rest
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
         if (p < n) S.sbody[p] = make_float4(bq[j].x, bq[j].y, bm[j], 0.0f);
     }
+}
+
+__global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
+                                                           const TileRef* __restrict__ ttiles,
+                                                           uint32_t* __restrict__ tsum) {
+    topo_pre_body(sims, ttiles, tsum, blockIdx.x, gridDim.x);
 }
 
 // Inclusive wavefront scan with max / min (all lanes active).
@@ -1133,11 +1164,12 @@ struct TileBounds {
     }
 };
 
-__global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __restrict__ sims,
-                                                             const TileRef* __restrict__ ttiles,
-                                                             uint32_t* __restrict__ ctl,
-                                                             unsigned long long* __restrict__ tflags,
-                                                             const uint32_t* __restrict__ psum) {
+__device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims,
+                                                const TileRef* __restrict__ ttiles,
+                                                uint32_t* __restrict__ ctl,
+                                                unsigned long long* __restrict__ tflags,
+                                                const uint32_t* __restrict__ psum, uint32_t block,
+                                                uint32_t grid) {
     // Each thread handles positions base + j * kTopoThreads + threadIdx.x, j < kTopoPer,
     // so a wavefront covers 64 consecutive positions in every slice j and the
     // per-block latency chain (ticket, tile, look-back) is paid once per 1024
@@ -1153,11 +1185,11 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
     // simulation from the block index, tile from a ticket of that simulation
     // (as in k_sort_pass)
-    const uint32_t sim = ttiles[blockIdx.x].sim;  // (an XCD-contiguous order measured slower here)
+    const uint32_t sim = ttiles[block].sim;  // (an XCD-contiguous order measured slower here)
     const SimDesc& S = sim_at(sims, sim);
     if (threadIdx.x == 0) s_ticket = S.first_topo_tile + atomicAdd(&S.meta->ticket[4], 1u);
     __syncthreads();
-    const uint32_t t = min(s_ticket, (uint32_t)(gridDim.x - 1u));
+    const uint32_t t = min(s_ticket, grid - 1u);
     const TileRef tr = ttiles[t];
     if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
         if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
@@ -1329,6 +1361,14 @@ __global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __re
     }
 }
 
+__global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __restrict__ sims,
+                                                             const TileRef* __restrict__ ttiles,
+                                                             uint32_t* __restrict__ ctl,
+                                                             unsigned long long* __restrict__ tflags,
+                                                             const uint32_t* __restrict__ psum) {
+    topo_count_body(sims, ttiles, ctl, tflags, psum, blockIdx.x, gridDim.x);
+}
+
 // Internal-cell aggregate from its children in quadrant order
 // (fill_aggregates internal branch, flat.rs:196-213).
 __device__ __forceinline__ Agg combine_children(const Agg* __restrict__ cagg, uint4 ch) {
@@ -1403,13 +1443,14 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
 constexpr uint32_t kLinkCells = 1536;
 constexpr uint32_t kLinkBatch = 4;  // cells of a column whose loads are issued together
 
-__global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
-                                                         const TileRef* __restrict__ ttiles) {
+__device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
+                                               const TileRef* __restrict__ ttiles, uint32_t block,
+                                               uint32_t grid) {
     __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];
     __shared__ uint32_t s_ch[kLinkCells][4];
     __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];
     __shared__ uint32_t s_cb0, s_cb1, s_dlo, s_dhi;
-    const TileRef tr = ttiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const TileRef tr = ttiles[xcd_tile(block, grid)];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
     const uint32_t p = tr.base + threadIdx.x;
@@ -1560,6 +1601,42 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
             link_write_cell(S, idx, nb + k, p, dmin + k, in, a, s_e[r], s_w[r], s_ch[r], in ? kNone : slot);
             if (!in) ++slot;
         }
+    }
+}
+
+__global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
+                                                         const TileRef* __restrict__ ttiles) {
+    topo_link_body(sims, ttiles, blockIdx.x, gridDim.x);
+}
+
+// The whole topology of a set with a single topology tile (one simulation of
+// at most kTopoTile bodies) in one workgroup: boundary pass, the scan of one
+// tile (no boundary before it, none after it but n), columns and cells,
+// links and aggregates, then the error gather of k_agg_top.  No cell crosses
+// a tile edge, so there are no spanning cells to aggregate.  Barriers between
+// the phases also order their global stores before the next phase's loads.
+__global__ __launch_bounds__(kTopoTile) void k_topo_single(const SimDesc* __restrict__ sims,
+                                                           const TileRef* __restrict__ ttiles,
+                                                           uint32_t* __restrict__ ctl,
+                                                           unsigned long long* __restrict__ tflags,
+                                                           uint32_t* __restrict__ tsum,
+                                                           uint32_t* __restrict__ psum,
+                                                           uint32_t* __restrict__ errs) {
+    const SimDesc& S = sim_at(sims, ttiles[0].sim);
+    topo_pre_body(sims, ttiles, tsum, 0u, 1u);
+    if (threadIdx.x < kLevels) {
+        psum[threadIdx.x] = 0u;
+        psum[kLevels + threadIdx.x] = S.n;
+    }
+    __syncthreads();
+    topo_count_body(sims, ttiles, ctl, tflags, psum, 0u, 1u);
+    __syncthreads();
+    topo_link_body(sims, ttiles, 0u, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (S.meta->nspan != 0u) atomicOr(&S.meta->err, kErrSpanCapacity);  // cannot happen
+        errs[0] = ctl[kCtlError];
+        errs[1] = S.meta->err;
     }
 }
 

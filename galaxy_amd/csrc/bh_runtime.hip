@@ -336,6 +336,14 @@ struct SimSet {
     void sort(hipStream_t s) {
         const dim3 grid((uint32_t)tsort.size()), block(kSortThreads);
         const uint32_t nt = (uint32_t)tsort.size();
+        if (nt == 1) {  // one tile: the four passes in one launch
+            if (sort_kpt == 8)
+                hipLaunchKernelGGL((k_sort_single<8>), dim3(1), block, 0, s, d_desc, d_tsort, d_ctl, d_sflags);
+            else
+                hipLaunchKernelGGL((k_sort_single<4>), dim3(1), block, 0, s, d_desc, d_tsort, d_ctl, d_sflags);
+            HIP_OK(hipGetLastError());
+            return;
+        }
         for (uint32_t pass = 0; pass < 4; ++pass) {
             if (sort_kpt == 8)
                 hipLaunchKernelGGL((k_sort_pass<8>), grid, block, 0, s, d_desc, d_tsort, nt, pass, d_ctl, d_sflags);
@@ -344,7 +352,16 @@ struct SimSet {
             HIP_OK(hipGetLastError());
         }
     }
+    // A single simulation of at most one topology tile: the whole topology and
+    // the error gather in one launch (k_topo_single), no k_agg_top.
+    bool single_topo() const { return desc.size() == 1 && ttopo.size() == 1; }
     void topo(hipStream_t s) {
+        if (single_topo()) {
+            hipLaunchKernelGGL(k_topo_single, dim3(1), dim3(kTopoTile), 0, s, d_desc, d_ttopo, d_ctl, d_tflags,
+                               d_tsum, d_psum, d_errs);
+            HIP_OK(hipGetLastError());
+            return;
+        }
         hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
                            d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
@@ -359,6 +376,7 @@ struct SimSet {
         HIP_OK(hipGetLastError());
     }
     void agg(hipStream_t s) {
+        if (single_topo()) return;  // done by k_topo_single
         hipLaunchKernelGGL(k_agg_top, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s, d_desc, d_ctl,
                            d_errs);
         HIP_OK(hipGetLastError());
