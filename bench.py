@@ -27,7 +27,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -187,34 +186,6 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
                 one_group=one_group)
 
 
-def serving_workload(requests=20000, rate_lo=50.0, rate_hi=2000.0, seed=777):
-    """The C4 request stream: Poisson arrivals whose rate ramps linearly from rate_lo
-    to rate_hi over requests / mean-rate seconds (19.5 s for 20,000), N log-uniform in
-    [1,024, 262,144], k uniform in {10, 100, 1,000}, initial conditions alternating
-    between a rotating disc (C0 physics) and a cold collapse (C1 physics), seed
-    10^6 + request index.  Returns [(arrival s, kind, n, seed, steps)]."""
-    rng = np.random.default_rng(seed)
-    span = requests / ((rate_lo + rate_hi) / 2.0)
-    t, out = 0.0, []
-    for k in range(requests):
-        rate = rate_lo + (rate_hi - rate_lo) * min(t / span, 1.0)
-        t += rng.exponential(1.0 / rate)
-        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(262144)))))
-        steps = int(rng.choice([10, 100, 1000]))
-        out.append((t, "disc" if k % 2 == 0 else "collapse", n, 10**6 + k, steps))
-    return out
-
-
-def request_physics(kind):
-    from galaxy_amd import ics
-
-    if kind == "disc":
-        w = ics.config0(n=2, steps=0)
-    else:
-        w = ics.config1(n=2, steps=0)
-    return w.config, w.dt
-
-
 def serving_run(gpu, requests=20000, choice=None, threads=16):
     """The C4 stream through the server's worker thread (continuous batching).
     Request states are generated before the clock starts (the clients' inputs);
@@ -225,14 +196,12 @@ def serving_run(gpu, requests=20000, choice=None, threads=16):
     from galaxy_amd import ics
     from galaxy_amd.serve import Server
 
-    reqs = serving_workload(requests)
-    phys = {k: request_physics(k) for k in ("disc", "collapse")}
+    reqs = ics.config4_requests(requests)
+    phys = {k: ics.request_physics(k) for k in ("disc", "collapse")}
 
     def make(r):
         _, kind, n, seed, _ = r
-        if kind == "disc":
-            return ics.rotating_disc(n, seed, softening=phys["disc"][0].softening_kpc)
-        return ics.uniform_collapse(n, seed)
+        return ics.request_state(kind, n, seed)
 
     t_gen = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:

@@ -14,8 +14,8 @@ Workloads (seeded, as the benchmark builds them):
   c1  cold collapse N=65,536, 1,000 steps, drop-in
   c2  binary disc encounter N=1,048,576, 500 steps, beyond-cap drop-in
   c3  256 rotating discs, N log-uniform in [512, 65,536], 500 steps, batched
-  c4  a sample of the serving stream (discs and collapses, N log-uniform in
-      [1,024, 262,144], k in {10, 100, 1,000}), run through the server
+  c4  the first 200 requests of the serving stream (discs and collapses, N
+      log-uniform in [1,024, 262,144], k in {10, 100, 1,000}), run through the server
 
 The f64 direct sums and the O(N^2) potential run on the GPU (gbh_diag_direct,
 gbh_diag_energy); both equal the CPU restatement bit for bit (tests/test_gpu_diag.py).
@@ -26,7 +26,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import sys
 import time
 
@@ -136,21 +135,15 @@ def run_c3(gpu, scale, count=256):
             "rel_lz_drift": summarize(rows, lambda r: r["rel_lz_drift"])}
 
 
-def run_c4(gpu, scale, requests=40, seed=777):
-    rng = np.random.default_rng(seed)
+def run_c4(gpu, scale, requests=200):
+    """The first `requests` requests of the C4 stream, submitted at once and
+    served to completion (their accuracy does not depend on arrival times)."""
     srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10)
     reqs = {}
-    for k in range(requests):
-        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(262144)))))
-        steps = max(1, int(int(rng.choice([10, 100, 1000])) * scale))
-        if k % 2 == 0:
-            s = ics.rotating_disc(n, 10**6 + k, softening=0.01)
-            cfg = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
-            dt = 1e-3
-        else:
-            s = ics.uniform_collapse(n, 10**6 + k)
-            cfg = Config(theta=0.5, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)
-            dt = 5e-4
+    for _, kind, n, seed, k in ics.config4_requests()[:requests]:
+        steps = max(1, int(k * scale))
+        s = ics.request_state(kind, n, seed)
+        cfg, dt = ics.request_physics(kind)
         rid = srv.submit(s, cfg, dt, steps)
         reqs[rid] = (s, cfg, steps)
     done = srv.drain()

@@ -5,7 +5,7 @@ The five measured configurations:
   C1  cold collapse, uniform unit disc, N=65,536, seed 303, eps=0.005, dt=5e-4, 1000 steps, bucket 4
   C2  binary disc encounter, 2 x 524,288 bodies, masses 1 : 0.5, eps=0.002, theta 0.6, dt=2.5e-4
   C3  256 rotating discs, N log-uniform in [512, 65536], scale length U[0.5, 2], seeds 1000+i
-  C4  serving stream of 'advance k steps' requests (see galaxy_amd.serve)
+  C4  serving stream of 20,000 'advance k steps' requests (config4_requests)
 Every generator returns frozen GpuEvolveState records, shape (n, 8) float32.
 """
 from __future__ import annotations
@@ -137,3 +137,36 @@ def config3(count=256, steps=500, bucket=4) -> Workload:
     cfg = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=bucket, max_depth=16)
     return Workload(f"C3 ensemble {count} discs", states, cfg, 1e-3, steps,
                     extra={"sizes": sizes, "scales": scales})
+
+
+def config4_requests(count=20000, rate_lo=50.0, rate_hi=2000.0, seed=777):
+    """The C4 request stream: Poisson arrivals whose rate ramps linearly from
+    rate_lo to rate_hi over count / mean-rate seconds (19.5 s for 20,000), N
+    log-uniform in [1,024, 262,144], k uniform in {10, 100, 1,000}, initial
+    conditions alternating between a rotating disc (C0 physics: eps 0.01, dt 1e-3,
+    leaf size 1) and a cold collapse (C1 physics: eps 0.005, dt 5e-4, leaf size 4),
+    seed 10^6 + request index.  Returns [(arrival s, kind, n, seed, steps)]; the
+    states come from request_state."""
+    rng = np.random.default_rng(seed)
+    span = count / ((rate_lo + rate_hi) / 2.0)
+    t, out = 0.0, []
+    for k in range(count):
+        rate = rate_lo + (rate_hi - rate_lo) * min(t / span, 1.0)
+        t += rng.exponential(1.0 / rate)
+        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(262144)))))
+        steps = int(rng.choice([10, 100, 1000]))
+        out.append((t, "disc" if k % 2 == 0 else "collapse", n, 10**6 + k, steps))
+    return out
+
+
+def request_physics(kind):
+    """(Config, dt) of a C4 request of the given kind ("disc": C0's, "collapse": C1's)."""
+    w = config0(n=2, steps=0) if kind == "disc" else config1(n=2, steps=0)
+    return w.config, w.dt
+
+
+def request_state(kind, n, seed) -> np.ndarray:
+    """Initial state of a C4 request."""
+    if kind == "disc":
+        return rotating_disc(n, seed, softening=request_physics("disc")[0].softening_kpc)
+    return uniform_collapse(n, seed)
