@@ -27,7 +27,11 @@ RECEIPT_SCHEMA = "galaxy.barnes-hut-parallel-gpu-tree-receipt.v1"
 BH2C_CAP = 4096
 TREE_LEVELS = 17
 TREE_BUCKET_SIZE = 4
-TREE_ORDERING = "stable-lsd-radix-4bit-morton-code; resident body index retained for equal Morton keys"
+# Frozen labels: the validator of scripts/bench-bh2d-hardware.py:40-55 demands these
+# strings verbatim.  They name the reference's tree (4-bit LSD radix, bind groups),
+# not the kernels that produce this receipt (8-bit onesweep passes, HIP graphs); the
+# ordering and layout they define are what the receipt's evidence follows.
+TREE_ORDERING ="stable-lsd-radix-4bit-morton-code; resident body index retained for equal Morton keys"
 TREE_LAYOUT = "sparse-level-order-slot=depth*N+group_start"
 BENCHMARK_SAMPLE_SCOPE = ("complete rebuild call including host-side uniform/bind-group setup, "
                           "command encoding, submit and synchronization")
