@@ -80,21 +80,31 @@ struct __attribute__((aligned(32))) CellT {
 // thresholds do not follow the preorder (general mode).
 struct __attribute__((aligned(16))) NodeHot {
     float x, y, m;
-    uint32_t w;  // ((subtree size - 1) << kSizeShift) | (code << 2)
+    uint32_t w;  // subtree size - 1 and threshold code (node_word)
 };
 struct __attribute__((aligned(16))) NodeCold {
     uint32_t start, count;
     float T;
     uint32_t tag;  // kTagBody | body index, or the depth
 };
+// Two layouts of the hot word, chosen per simulation (SimDesc::wlayout):
+//  - 0: ((size - 1) << 6) | (code << 2), for any node count below kMaxNodes;
+//  - 1 (device trees of at most 2^22 node records): ((size - 1) << 10) |
+//    (code == 0 ? 1 << 9 : 0) | (code << 2).  Then w << 22 is the opening
+//    test's exponent offset: code << 24 for the depth codes and 2^31 for code 0
+//    (bodies and the root, threshold 0), which every d2 >= 0 passes (see
+//    pow4_thresholds).
+// Both keep the code-table byte offset in bits 2..5.  The word holds size - 1,
+// so a walk advances by at least one node per step whatever the word holds.
 constexpr uint32_t kSizeShift = 6;
+constexpr uint32_t kSizeShiftNarrow = 10;
+constexpr uint32_t kNarrowNodes = 1u << (32 - kSizeShiftNarrow);
 constexpr uint32_t kCodes = 16;
 constexpr uint32_t kCodeField = (kCodes - 1u) << 2;          // byte offset into the code table
 constexpr uint32_t kMaxNodes = 1u << (32 - kSizeShift);      // subtree sizes fit the hot word
-// The word holds size - 1, so a walk advances by at least one node per step
-// whatever the word holds.
-__host__ __device__ constexpr uint32_t node_word(uint32_t size, uint32_t code) {
-    return ((size - 1u) << kSizeShift) | (code << 2);
+__host__ __device__ constexpr uint32_t node_word(uint32_t size, uint32_t code, uint32_t layout = 0u) {
+    return layout != 0u ? (((size - 1u) << kSizeShiftNarrow) | (code == 0u ? 1u << 9 : 0u) | (code << 2))
+                        : (((size - 1u) << kSizeShift) | (code << 2));
 }
 constexpr uint32_t kTagBody = 1u << 31;
 constexpr uint32_t kTagFirst = 1u << 30;  // first body of its leaf
@@ -128,7 +138,8 @@ struct __attribute__((aligned(16))) SimDesc {
     float theta, eps, G, dt;
     uint32_t first_tile, first_sort_tile, first_topo_tile, span_cap;
     uint32_t node_cap;  // node records allocated (the traversal reads at most index node_cap - 1)
-    uint32_t pad_[3];
+    uint32_t wlayout;   // hot-word layout (node_word): 1 when node_cap <= kNarrowNodes (device trees)
+    uint32_t pad_[2];
     // resident state (SoA)
     float2* pos;
     float* mass;
@@ -1328,7 +1339,7 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
                         wx = wx + bm * (double)bq.x;
                         wy = wy + bm * (double)bq.y;
                         const uint32_t ni = nfirst + (q - p);
-                        S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u)};
+                        S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u, S.wlayout)};
                         S.bnode[q] = ni;
                     }
                 }
@@ -1402,7 +1413,7 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
                                                 bool inside, const Agg& a, uint32_t e, uint32_t w,
                                                 const uint32_t* lds_ch, uint32_t slot = kNone) {
     const float T = S.meta->T[d];
-    if (w == 0u) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d);
+    if (w == 0u) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d, S.wlayout);
     if (inside) {
         if (lds_ch) S.cchild[idx] = make_uint4(lds_ch[0], lds_ch[1], lds_ch[2], lds_ch[3]);
         S.cagg[idx] = a;
@@ -1524,7 +1535,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
                 S.ncold[ni] = NodeCold{p, e - p, tv[j], d};
                 if (e <= tile_end) inside |= 1u << k;
                 if (in_lds) {
-                    s_w[idx - cb0] = node_word(sk[j] - ni, d);  // internal depths are 0..15
+                    s_w[idx - cb0] = node_word(sk[j] - ni, d, S.wlayout);  // internal depths are 0..15
                     s_e[idx - cb0] = e;
                     s_ch[idx - cb0][(kp >> (30u - 2u * d)) & 3u] = idx + 1u;
                 }
@@ -1885,31 +1896,43 @@ struct WalkEvidence {
 // The walk is software-pipelined: the next node's gather is issued before the
 // interaction of the current one is evaluated.  Whatever the node words hold,
 // every step advances the cursor by at least one node, so a walk always ends.
-// The hot records are read through a buffer resource whose size is the node
-// array's: a gather beyond it (the early gather past the last node, whose value
-// is never used) returns zeros instead of touching memory, with no clamp or
-// 64-bit address arithmetic in the loop.  `tab` is the byte address of the
-// wavefront's code table in LDS.
-// Buffer resource over `records` 16-byte node records (raw buffer, 32-bit
-// offsets; loads beyond the size return zeros).  gfx9 raw-buffer word 3.
-constexpr int kRawBufferWord3 = 0x00020000;
+// The hot records are read through a structured buffer resource (16-byte
+// records, the record index in the instruction: no address arithmetic in the
+// loop) whose size is the node stream's: a gather at or beyond the stream's end
+// (the early gather past the last node) returns zeros instead of touching
+// memory.  `tab` is the byte address of the wavefront's code table in LDS.
+//
+// A zero record is a body of mass 0 at the origin with subtree size 1.  In the
+// fast walks (no evidence) a lane whose cursor has reached the end may take one
+// more step on it, so the end test runs once per two steps: its interaction adds
+// G 0 r^-3 d = +-0 to the lane's sums (r^-3 is finite there, as for every node:
+// positions lie within 1e18 and 0 < eps2), which leaves them unchanged because a
+// sum that starts at +0 is never -0, and the cursor moves on by one.
+constexpr int kRawBufferWord3 = 0x00020000;  // gfx9 buffer word 3 (32-bit data format)
+constexpr short kNodeStride = 16;
 // The arguments must be wave-uniform (they are made explicitly so, which keeps
 // the resource in scalar registers).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t node_buffer(const void* p, uint32_t records) {
     const uint64_t a = (uint64_t)(uintptr_t)p;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    const uint32_t bytes = __builtin_amdgcn_readfirstlane(records * 16u);
+    const uint32_t nrec = __builtin_amdgcn_readfirstlane(records);
     void* base = (void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
-    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, kRawBufferWord3);
+    return __builtin_amdgcn_make_buffer_rsrc(base, kNodeStride, (int)nrec, kRawBufferWord3);
 }
+// llvm.amdgcn.struct.ptr.buffer.load (buffer_load_dwordx4 ... idxen): record
+// `vindex` of the resource, `voffset` bytes into it.
+__device__ F4v struct_buffer_load_v4f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                        int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
 __device__ __forceinline__ F4v load_node(__amdgpu_buffer_rsrc_t r, uint32_t index) {
-    return __builtin_bit_cast(F4v, __builtin_amdgcn_raw_buffer_load_b128(r, index * 16u, 0, 0));
+    return struct_buffer_load_v4f32(r, (int)index, 0, 0, 0);
 }
 
-// kPow4: the thresholds scale by powers of four (SimMeta::pow4), so the opening
-// test is an integer comparison against T[0] (pow4_thresholds) instead of a
-// code-table read from LDS; code 0 (bodies, the root) has threshold 0.
+// kPow4: the thresholds scale by powers of four (SimMeta::pow4) and the words
+// have layout 1, so the opening test is one integer comparison against T[0]
+// (pow4_thresholds): bits(d2) + (w << 22) >= bits(T[0]), where w << 22 is the
+// code's exponent offset (node_word).  Otherwise the threshold comes from the
+// code table in LDS and the size from the layout's shift.
 template <bool kEvidence, bool kGeneral, bool kFast, bool kPow4 = false>
 struct NodeWalk {
     __amdgpu_buffer_rsrc_t nh;
@@ -1917,6 +1940,7 @@ struct NodeWalk {
     uint32_t tab, nn, p, self;
     float tx, ty, eps2, G;
     uint32_t t0_bits = 0;
+    uint32_t wshift = kSizeShift;  // the word layout's size shift
     float ax = 0.f, ay = 0.f;
     WalkEvidence ev;
 
@@ -1929,7 +1953,7 @@ struct NodeWalk {
         const float dd = dx * dx;
         const float ee = dy * dy;
         const float d2 = dd + ee;
-        const uint32_t size_m1 = w >> kSizeShift;  // subtree [c, c + size_m1]
+        const uint32_t size_m1 = w >> (kPow4 ? kSizeShiftNarrow : wshift);  // subtree [c, c + size_m1]
         bool inside;
         bool open_ok;  // d2 >= the node's threshold
         U4v l;
@@ -1939,8 +1963,7 @@ struct NodeWalk {
             open_ok = d2 >= __uint_as_float(l.z);
         } else if (kPow4) {
             inside = (self - c) <= size_m1;
-            const uint32_t code4 = w & kCodeField;  // code << 2
-            open_ok = code4 == 0u || __float_as_uint(d2) + (code4 << 22) >= t0_bits;
+            open_ok = __float_as_uint(d2) + (w << 22) >= t0_bits;
         } else {
             inside = (self - c) <= size_m1;
             open_ok = d2 >= *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
@@ -1969,12 +1992,15 @@ struct NodeWalk {
         return next;
     }
 
-    // Two steps per trip with the records alternating between g0 and g1.
+    // Two steps per trip with the records alternating between g0 and g1 (the
+    // fast walks test the end once per trip: see the zero record above).
     __device__ __forceinline__ void run(uint32_t c) {
         F4v g0 = load_node(nh, c), g1;
         while (c < nn) {
             c = step(c, g0, g1);
-            if (c >= nn) break;
+            if (kGeneral || kEvidence || !kFast) {
+                if (c >= nn) break;
+            }
             c = step(c, g1, g0);
         }
     }
@@ -2064,6 +2090,8 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const uint32_t geom_ok = __builtin_amdgcn_readfirstlane(S.meta->geom_ok);
         const uint32_t pow4 = __builtin_amdgcn_readfirstlane(S.meta->pow4);
         const uint32_t t0_bits = __builtin_amdgcn_readfirstlane(S.meta->t0_bits);
+        const uint32_t narrow = __builtin_amdgcn_readfirstlane(S.wlayout);
+        const uint32_t wshift = narrow != 0u ? kSizeShiftNarrow : kSizeShift;
         const float eps = S.eps;
         const float eps2 = eps * eps;
         const float G = S.G;
@@ -2082,23 +2110,22 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         float ax = 0.f, ay = 0.f;
         WalkEvidence ev;
         const uint32_t c0 = valid ? 0u : nn;
-        // node_cap < 2^26 records, so the byte size fits the resource's 32 bits
-        const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, S.node_cap);
+        const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, nn);
         const GlobalU4 nc = (GlobalU4)(const void*)S.ncold;
         if (general) {
-            NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
             wk.run(c0);
             ax = wk.ax, ay = wk.ay, ev = wk.ev;
-        } else if (fast && pow4 && !kEvidence) {
+        } else if (fast && pow4 && narrow != 0u && !kEvidence) {
             NodeWalk<kEvidence, false, true, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, t0_bits};
             wk.run(c0);
             ax = wk.ax, ay = wk.ay, ev = wk.ev;
         } else if (fast) {
-            NodeWalk<kEvidence, false, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            NodeWalk<kEvidence, false, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
             wk.run(c0);
             ax = wk.ax, ay = wk.ay, ev = wk.ev;
         } else {
-            NodeWalk<kEvidence, false, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G};
+            NodeWalk<kEvidence, false, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
             wk.run(c0);
             ax = wk.ax, ay = wk.ay, ev = wk.ev;
         }

@@ -125,6 +125,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cell_cap = (uint32_t)cap;
     d.span_cap = scap;
     d.node_cap = (uint32_t)(cap + n);
+    d.wlayout = d.node_cap <= kNarrowNodes ? 1u : 0u;
     return align_up(off, 256);
 }
 
