@@ -162,8 +162,8 @@ struct __attribute__((aligned(16))) SimDesc {
     uint32_t* cend;    // end of each cell's range (written by k_topo_count)
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
-    uint32_t* span;    // spanning internal cells (span_cap entries)
-    uint32_t* spanbuf; // the same bucketed by depth (k_agg_top, listed mode)
+    uint2* span;       // spanning internal cells (span_cap entries): (cell, depth << 26 | node)
+    uint32_t* spanbuf; // their slots bucketed by depth (k_agg_top, listed mode)
     NodeHot* nodes;    // traversal node stream (cell_cap + n entries)
     NodeCold* ncold;   // its cold records
     uint32_t* nbase;   // per sorted position: node index of the column's first cell
@@ -1423,7 +1423,7 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
         uint32_t info = d;
         if (slot == kNone) slot = atomicAdd(&S.meta->nspan, 1u);
         if (slot < S.span_cap) {
-            S.span[slot] = idx;
+            S.span[slot] = make_uint2(idx, (d << 26) | ni);  // node indices are below kMaxNodes = 2^26
             info |= (slot + 1u) << kInfoSpanShift;
         } else {
             atomicOr(&S.meta->err, kErrSpanCapacity);
@@ -1626,13 +1626,11 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restri
 // links and aggregates, then the error gather of k_agg_top.  No cell crosses
 // a tile edge, so there are no spanning cells to aggregate.  Barriers between
 // the phases also order their global stores before the next phase's loads.
-__global__ __launch_bounds__(kTopoTile) void k_topo_single(const SimDesc* __restrict__ sims,
-                                                           const TileRef* __restrict__ ttiles,
-                                                           uint32_t* __restrict__ ctl,
-                                                           unsigned long long* __restrict__ tflags,
-                                                           uint32_t* __restrict__ tsum,
-                                                           uint32_t* __restrict__ psum,
-                                                           uint32_t* __restrict__ errs) {
+__device__ __forceinline__ void topo_single_body(const SimDesc* __restrict__ sims,
+                                                 const TileRef* __restrict__ ttiles, uint32_t* __restrict__ ctl,
+                                                 unsigned long long* __restrict__ tflags,
+                                                 uint32_t* __restrict__ tsum, uint32_t* __restrict__ psum,
+                                                 uint32_t* __restrict__ errs) {
     const SimDesc& S = sim_at(sims, ttiles[0].sim);
     topo_pre_body(sims, ttiles, tsum, 0u, 1u);
     if (threadIdx.x < kLevels) {
@@ -1649,6 +1647,191 @@ __global__ __launch_bounds__(kTopoTile) void k_topo_single(const SimDesc* __rest
         errs[0] = ctl[kCtlError];
         errs[1] = S.meta->err;
     }
+}
+
+__global__ __launch_bounds__(kTopoTile) void k_topo_single(const SimDesc* __restrict__ sims,
+                                                           const TileRef* __restrict__ ttiles,
+                                                           uint32_t* __restrict__ ctl,
+                                                           unsigned long long* __restrict__ tflags,
+                                                           uint32_t* __restrict__ tsum,
+                                                           uint32_t* __restrict__ psum,
+                                                           uint32_t* __restrict__ errs) {
+    topo_single_body(sims, ttiles, ctl, tflags, tsum, psum, errs);
+}
+
+// The whole build of a set of one simulation of at most kTopoTile bodies in one
+// workgroup, one body per thread: the kick/drift (mode as in k_bounds) and the
+// resets of k_bounds, root bounds, thresholds and Morton keys as in k_keys, the
+// four stable radix passes with the ranking of sort_pass_body held in LDS (no
+// look-back: the set has one tile), the sorted keys and indices to key0/val0,
+// then the topology of k_topo_single.  The sort leaves the per-pass digit
+// histograms and look-back flags unused.
+__global__ __launch_bounds__(kTopoTile) void k_build_small(const SimDesc* __restrict__ sims,
+                                                           const TileRef* __restrict__ ttiles,
+                                                           uint32_t* __restrict__ ctl,
+                                                           unsigned long long* __restrict__ tflags,
+                                                           uint32_t* __restrict__ tsum,
+                                                           uint32_t* __restrict__ psum,
+                                                           uint32_t* __restrict__ errs, int mode,
+                                                           uint32_t clear_errors) {
+    constexpr uint32_t kWaves = kTopoTile / 64;
+    __shared__ float s_red[4][kWaves];
+    __shared__ double s_root[3];
+    __shared__ uint32_t s_key[kTopoTile], s_val[kTopoTile];
+    __shared__ uint32_t s_wcnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_scan[kWaves];
+    const SimDesc& S = sim_at(sims, ttiles[0].sim);
+    const uint32_t n = S.n;
+    const uint32_t i = threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t w = i >> 6;
+    // kick/drift and bounds (k_bounds), the body's loads first
+    const uint32_t ic = min(i, n - 1u);
+    float2 p = S.pos[ic];
+    float2 v = make_float2(0.f, 0.f), a = make_float2(0.f, 0.f);
+    if (mode != 0) {
+        v = S.vel[ic];
+        a = S.acc[ic];
+    }
+    if (i < kXcds) ctl[kCtlQueue0 + i * kQueueStride] = 0u;
+    if (i == 0) tflags[0] = 0ull;
+    if (i < 8) S.meta->ticket[i] = 0u;
+    if (clear_errors && i == 0) {
+        S.meta->err = 0u;
+        ctl[kCtlError] = 0u;
+    }
+    if (valid && mode != 0) {
+        const float dt = S.dt;
+        const float h = 0.5f * dt;
+        if (mode == 2) {  // bh_final_kick of the previous step
+            v.x = v.x + h * a.x;
+            v.y = v.y + h * a.y;
+        }
+        const float hvx = v.x + h * a.x;  // bh_kick_drift
+        const float hvy = v.y + h * a.y;
+        p.x = p.x + dt * hvx;
+        p.y = p.y + dt * hvy;
+        S.pos[i] = p;
+        S.vel[i] = make_float2(hvx, hvy);
+    }
+    float mnx = valid ? p.x : __builtin_inff(), mny = valid ? p.y : __builtin_inff();
+    float mxx = valid ? p.x : -__builtin_inff(), mxy = valid ? p.y : -__builtin_inff();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = fminf(mnx, __shfl_xor(mnx, off));
+        mny = fminf(mny, __shfl_xor(mny, off));
+        mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+        mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+    }
+    if (lane_id() == 0) {
+        s_red[0][w] = mnx;
+        s_red[1][w] = mny;
+        s_red[2][w] = mxx;
+        s_red[3][w] = mxy;
+    }
+    __syncthreads();
+    // root_bounds in f64 over the widened f32 positions (min/max are exact, so
+    // the order of the reduction does not matter)
+    if (i == 0) {
+        for (uint32_t k = 1; k < kWaves; ++k) {
+            mnx = fminf(mnx, s_red[0][k]);
+            mny = fminf(mny, s_red[1][k]);
+            mxx = fmaxf(mxx, s_red[2][k]);
+            mxy = fmaxf(mxy, s_red[3][k]);
+        }
+        double min_x = mnx, min_y = mny, max_x = mxx, max_y = mxy;
+        double sx = max_x - min_x, sy = max_y - min_y;
+        double span = sy > sx ? sy : sx;
+        span = 1e-12 > span ? 1e-12 : span;
+        s_root[0] = (min_x + max_x) * 0.5;
+        s_root[1] = (min_y + max_y) * 0.5;
+        s_root[2] = span * 0.500000000001 + 1e-12;
+    }
+    __syncthreads();
+    const double cx = s_root[0], cy = s_root[1], half = s_root[2];
+    if (w == 0) {
+        SimMeta* M = S.meta;
+        if (i == 0) {
+            M->root[0] = cx;
+            M->root[1] = cy;
+            M->root[2] = half;
+            M->root[3] = 0.0;
+            M->bounds_f32[0] = (float)cx;
+            M->bounds_f32[1] = (float)cy;
+            M->bounds_f32[2] = (float)half;
+            M->bounds_f32[3] = 0.0f;
+            M->nspan = 0u;
+            M->theta_bits = __float_as_uint(S.theta);
+            M->mode = 0u;
+            M->geom_ok = half < 1.0e18 ? 1u : 0u;
+        }
+        float T = 0.0f;
+        if (i < kLevels) {
+            double hd = half;
+            for (uint32_t l = 0; l < i; ++l) hd = hd * 0.5;
+            T = accept_threshold((float)hd, S.theta);
+            M->T[i] = T;
+            if (i < kCodes) M->tab[i] = i == 0 ? 0.0f : T;
+        }
+        const float T0 = __shfl(T, 0);
+        const uint32_t p4 = pow4_thresholds(T, T0);
+        if (i == 0) {
+            M->pow4 = p4;
+            M->t0_bits = __float_as_uint(T0);
+        }
+    }
+    // Morton keys, then four stable 8-bit passes: element k of the sequence is
+    // held by thread k (waves in order, lanes in order within a wave), ranked
+    // among its wavefront's equal digits by ballots, and placed after the
+    // smaller digits and after the earlier wavefronts' equal digits.
+    uint32_t key = 0u, val = i;
+    if (valid)
+        key = spread16(quantize_axis((double)p.x, cx, half)) | (spread16(quantize_axis((double)p.y, cy, half)) << 1);
+    const unsigned long long lt_mask = __lanemask_lt();
+    for (uint32_t pass = 0; pass < 4; ++pass) {
+        for (uint32_t k = i; k < kWaves * kRadix; k += kTopoTile) (&s_wcnt[0][0])[k] = 0u;
+        __syncthreads();
+        const uint32_t d = (key >> (8u * pass)) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && below == 0) s_wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t tot = 0;
+        if (i < kRadix) {
+            for (uint32_t k = 0; k < kWaves; ++k) {
+                const uint32_t c = s_wcnt[k][i];
+                s_wcnt[k][i] = tot;
+                tot += c;
+            }
+        }
+        uint32_t all;
+        const uint32_t start = block_excl_sum<kTopoTile>(tot, s_scan, all);
+        if (i < kRadix) s_start[i] = start;
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = s_start[d] + s_wcnt[w][d] + below;
+            s_key[dst] = key;
+            s_val[dst] = val;
+        }
+        __syncthreads();
+        if (valid) {
+            key = s_key[i];
+            val = s_val[i];
+        }
+    }
+    if (valid) {
+        S.key0[i] = key;
+        S.val0[i] = val;
+    }
+    __syncthreads();
+    topo_single_body(sims, ttiles, ctl, tflags, tsum, psum, errs);
 }
 
 // ----------------------------------------------------------------------------
@@ -1706,7 +1889,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     __shared__ uint32_t s_ref[kTopLdsCells][4];
     __shared__ uint32_t s_idx[kTopLdsCells], s_node[kTopLdsCells];  // cell and node index
     __shared__ uint8_t s_depth[kTopLdsCells];
-    __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
+    __shared__ uint32_t s_next, s_dmask, s_cnt[kLevels], s_off[kLevels + 1];
     const SimDesc& S = sim_at(sims, blockIdx.x);
     if (threadIdx.x == 0) {
         errs[1 + blockIdx.x] = S.meta->err;
@@ -1714,25 +1897,41 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     }
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
+    constexpr uint32_t kNodeMask = (1u << 26) - 1u;
+    if (threadIdx.x == 0) s_dmask = 0u;
     if (nspan <= kTopLdsCells) {
         if (threadIdx.x == 0) s_next = nspan;
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-            const uint32_t idx = S.span[i];
-            const uint32_t depth = S.cells[idx].info & 255u;
-            const uint32_t st = S.cells[idx].start;
+            // three dependent gathers: the span entry, the cell's children, the
+            // children's span slots (then their aggregates), each level's loads
+            // issued together
+            const uint2 se = S.span[i];
+            const uint32_t idx = se.x, depth = se.y >> 26;
             s_idx[i] = idx;
             s_depth[i] = (uint8_t)depth;
-            s_node[i] = S.nbase[st] + (depth - S.cdmin[st]);
+            s_node[i] = se.y & kNodeMask;
+            atomicOr(&s_dmask, 1u << depth);
             const uint4 ch = S.cchild[idx];
+            uint32_t slot[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t c = (&ch.x)[q];
-                const uint32_t slot = c != kNoChild ? S.cells[c].info >> kInfoSpanShift : 0u;
-                const bool preload = c != kNoChild && slot == 0u;
+                slot[q] = c != kNoChild ? S.cells[c].info >> kInfoSpanShift : 0u;
+            }
+            Agg ca[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = (&ch.x)[q];
+                ca[q] = c != kNoChild && slot[q] == 0u ? S.cagg[c] : Agg{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = (&ch.x)[q];
+                const bool preload = c != kNoChild && slot[q] == 0u;
                 // preload slots: one LDS atomic per wavefront, ranks from the ballot
                 const unsigned long long m = __ballot(preload);
-                uint32_t r = c == kNoChild ? kNoChild : slot - 1u;
+                uint32_t r = c == kNoChild ? kNoChild : slot[q] - 1u;
                 if (m) {
                     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
                     uint32_t first = 0u;
@@ -1740,17 +1939,18 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
                     first = __shfl(first, (int)leader);
                     if (preload) {
                         r = first + (uint32_t)__builtin_popcountll(m & ((1ull << lane_id()) - 1ull));
-                        const Agg a = S.cagg[c];
-                        s_m[r] = a.m;
-                        s_x[r] = a.x;
-                        s_y[r] = a.y;
+                        s_m[r] = ca[q].m;
+                        s_x[r] = ca[q].x;
+                        s_y[r] = ca[q].y;
                     }
                 }
                 s_ref[i][q] = r;
             }
         }
         __syncthreads();
+        const uint32_t dmask = s_dmask;
         for (int d = 15; d >= 0; --d) {
+            if (!((dmask >> d) & 1u)) continue;  // (uniform) no spanning cell at this depth
             for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
                 if (s_depth[i] != (uint32_t)d) continue;
                 double mass = 0.0, wx = 0.0, wy = 0.0;
@@ -1774,11 +1974,11 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         }
         return;
     }
-    // listed mode: counting sort of the spanning cells by depth into spanbuf
+    // listed mode: counting sort of the span slots by depth into spanbuf
     if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
-        atomicAdd(&s_cnt[min(S.cells[S.span[i]].info & 255u, kLevels - 1u)], 1u);
+        atomicAdd(&s_cnt[min(S.span[i].y >> 26, kLevels - 1u)], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
@@ -1791,17 +1991,15 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-        const uint32_t idx = S.span[i];
-        const uint32_t d = min(S.cells[idx].info & 255u, kLevels - 1u);
-        S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = idx;
+        const uint32_t d = min(S.span[i].y >> 26, kLevels - 1u);
+        S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = i;
     }
     __syncthreads();  // (also orders the list's global stores before the rounds' loads)
     for (int d = 15; d >= 0; --d) {
+        if (s_off[d] == s_off[d + 1]) continue;  // (uniform) no spanning cell at this depth
         for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kTopThreads) {
-            const uint32_t idx = S.spanbuf[k];
-            const uint32_t st = S.cells[idx].start;
-            const uint32_t node = S.nbase[st] + ((uint32_t)d - S.cdmin[st]);
-            store_span(S, idx, node, agg_span_global(S, idx));
+            const uint2 se = S.span[S.spanbuf[k]];
+            store_span(S, se.x, se.y & kNodeMask, agg_span_global(S, se.x));
         }
         __syncthreads();  // the next depth reads this depth's global stores
     }

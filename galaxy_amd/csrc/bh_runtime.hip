@@ -114,7 +114,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cend = (uint32_t*)take(4ull * cap);
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
-    d.span = (uint32_t*)take(4ull * scap);
+    d.span = (uint2*)take(8ull * scap);
     d.spanbuf = (uint32_t*)take(4ull * scap);
     d.nodes = (NodeHot*)take(sizeof(NodeHot) * (cap + n));
     d.ncold = (NodeCold*)take(sizeof(NodeCold) * (cap + n));
@@ -387,6 +387,19 @@ struct SimSet {
         sort(s);
         topo(s);
         agg(s);
+    }
+    // Kick/drift (mode as in k_bounds) and the whole build, in launch order.  A
+    // single simulation of at most one topology tile is one launch
+    // (k_build_small); the per-phase launches above stay for phase timing.
+    void bounds_build(int mode, hipStream_t s, bool clear_errors = false) {
+        if (single_topo()) {
+            hipLaunchKernelGGL(k_build_small, dim3(1), dim3(kTopoTile), 0, s, d_desc, d_ttopo, d_ctl, d_tflags,
+                               d_tsum, d_psum, d_errs, mode, clear_errors ? 1u : 0u);
+            HIP_OK(hipGetLastError());
+            return;
+        }
+        bounds(mode, s, clear_errors);
+        build(s);
     }
     void force(hipStream_t s, bool reset_queue, uint32_t* ev_stats = nullptr,
                unsigned long long* ev_hash = nullptr, uint32_t ev_sim = 0) {
@@ -1019,14 +1032,14 @@ int gbh_tree_rebuild(gbh_runtime* rt, gbh_state* st, gbh_tree* t, const gbh_conf
         // The whole build is one HIP graph, captured at the first rebuild: bounds
         // (which also clears the previous rebuild's error words and the traversal
         // queue), keys, the radix passes, topology, top aggregates with the error
-        // gather, and the copy of the error words to pinned host memory.  The
-        // kernels read the configuration from the simulation table, so a changed
+        // gather, and the copy of the error words to pinned host memory (a tree
+        // of at most 1,024 bodies: one fused kernel and the copy).  The kernels
+        // read the configuration from the simulation table, so a changed
         // configuration needs no new capture.
         if (!t->rebuild_graph) {
             hipGraph_t gr = nullptr;
             HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            t->set.bounds(0, s, true);
-            t->set.build(s);
+            t->set.bounds_build(0, s, true);
             t->set.copy_errors(s);
             HIP_OK(hipStreamEndCapture(s, &gr));
             const hipError_t e = hipGraphInstantiate(&t->rebuild_graph, gr, nullptr, nullptr, 0);

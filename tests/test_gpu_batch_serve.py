@@ -65,6 +65,30 @@ def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile, block, waves, buck
     b.close()
 
 
+@pytest.mark.parametrize("n,use_graph", [(2, True), (700, True), (1024, False), (1025, True)])
+def test_batch_of_one_small_simulation(oracle, gpu, n, use_graph):
+    """A batch of one simulation of at most 1,024 bodies builds in one launch per
+    step, kick/drift and final kick included; 1,025 bodies take the tiled
+    kernels.  Both give the oracle's trajectory, and so does the per-phase step
+    that times the phases (separate launches)."""
+    cfg = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    s0 = ics.rotating_disc(n, 2000 + n, scale=0.7)
+    print(f"single-simulation batch n={n} graph={use_graph}", flush=True)
+    b = Batch(gpu, [s0], [cfg], [1e-3], use_graph=use_graph)
+    b.step(3)
+    b.phase_ms()  # one more step, phase by phase
+    b.step(1)
+    s, a = b.read(0)
+    rs, ra = oracle.f32_evolve(s0, 5, 1e-3, ocfg(oracle, cfg))
+    assert np.array_equal(s.view(np.uint32), rs.view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), ra.view(np.uint32))
+    st, hsh = b.force_evidence(0)
+    t = oracle.tree_from_state(s, ocfg(oracle, cfg))
+    _, ostats, ohash = oracle.traverse_f32(*t.pack(), evidence=True)
+    assert np.array_equal(st, ostats) and np.array_equal(hsh, ohash)
+    b.close()
+
+
 @pytest.mark.parametrize("use_graph,block,waves,streams", [(True, 256, 20, 1), (False, 64, 16, 1),
                                                           (True, 256, 16, 2), (False, 128, 20, 2)])
 def test_batch_many_sims_per_xcd_work(oracle, gpu, use_graph, block, waves, streams):
