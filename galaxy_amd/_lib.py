@@ -104,6 +104,7 @@ def lib():
         "gbh_tree_buffer_bytes": ([vp], u64),
         "gbh_tree_last_force_ms": ([vp], dbl),
         "gbh_check_rounding": ([vp, vp, vp], i32),
+        "gbh_check_thresholds": ([vp, vp, vp, u32, vp], i32),
         "gbh_diag_direct": ([vp, vp, u32, vp, vp, vp], i32),
         "gbh_diag_energy": ([vp, dbl, dbl, vp], i32),
         "gbh_copy_calibration": ([vp, u64, vp], i32),

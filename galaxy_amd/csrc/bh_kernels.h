@@ -163,7 +163,7 @@ struct __attribute__((aligned(16))) SimDesc {
     uint4* cchild;     // children (kNoChild when absent)
     Agg* cagg;         // f64 aggregates
     uint2* span;       // spanning internal cells (span_cap entries): (cell, depth << 26 | node)
-    uint32_t* spanbuf; // their slots bucketed by depth (k_agg_top, listed mode)
+    uint2* spanbuf;    // the same bucketed by depth (k_agg_top, listed mode)
     NodeHot* nodes;    // traversal node stream (cell_cap + n entries)
     NodeCold* ncold;   // its cold records
     uint32_t* nbase;   // per sorted position: node index of the column's first cell
@@ -394,11 +394,45 @@ __device__ float accept_threshold(float half, float theta) {
     if (!pass(inf_bits)) return __uint_as_float(0x7fc00000u);
     if (pass(1u)) return __uint_as_float(1u);
     uint32_t lo = 1u, hi = inf_bits;  // pass(lo) false, pass(hi) true
+    // pass() is monotone in d2 (a correctly rounded square root and quotient
+    // are), so the answer is the one bit pattern where it turns true.  Start
+    // next to (w / theta)^2 and step a few patterns either way, keeping the
+    // bracket; the bisection below closes what the steps leave open.
+    const double est = ((double)w / (double)theta) * ((double)w / (double)theta);
+    if (est > 0.0 && est < 3.0e38) {
+        const uint32_t b = min(max(__float_as_uint((float)est), lo + 1u), hi - 1u);
+        if (pass(b)) {
+            hi = b;
+            for (int k = 0; k < 4 && hi - lo > 1u; ++k) {
+                if (!pass(hi - 1u)) {
+                    lo = hi - 1u;
+                    break;
+                }
+                --hi;
+            }
+        } else {
+            lo = b;
+            for (int k = 0; k < 4 && hi - lo > 1u; ++k) {
+                if (pass(lo + 1u)) {
+                    hi = lo + 1u;
+                    break;
+                }
+                ++lo;
+            }
+        }
+    }
     while (hi - lo > 1u) {
         uint32_t mid = lo + ((hi - lo) >> 1);
         if (pass(mid)) hi = mid; else lo = mid;
     }
     return __uint_as_float(hi);
+}
+
+// Thresholds of given (half, theta) pairs (gbh_check_thresholds).
+__global__ void k_check_thresholds(const float* __restrict__ half, const float* __restrict__ theta,
+                                   uint32_t count, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = accept_threshold(half[i], theta[i]);
 }
 
 // Whether a simulation's opening thresholds scale by powers of four: T[0] is a
@@ -1837,35 +1871,54 @@ __global__ __launch_bounds__(kTopoTile) void k_build_small(const SimDesc* __rest
 // ----------------------------------------------------------------------------
 // K6: aggregates of the spanning internal cells (the top of the tree), one
 // workgroup per simulation, bottom-up by depth.  The footprint is small (256
-// threads, about 37 KB of LDS) so that these workgroups run beside the other
+// threads, about 40 KB of LDS) so that these workgroups run beside the other
 // group's persistent traversal instead of waiting for it to end.
 //  - Up to kTopLdsCells spanning cells ("LDS mode"): every global read happens
 //    before the depth rounds; each spanning cell's children become references
 //    to either another spanning cell's running value or a preloaded aggregate
 //    of a non-spanning child, all in LDS, so a round is an LDS round trip.
-//  - More ("listed mode"): the cells are bucketed by depth into the
+//  - More ("listed mode"): the span entries are bucketed by depth into the
 //    simulation's spanbuf (a counting sort), and each depth round aggregates its
-//    own bucket through global memory, children in quadrant order as ever.
+//    own bucket through global memory.
+// Each cell's children are summed in quadrant order, absent ones skipped
+// (fill_aggregates internal branch, flat.rs:196-213).  The loads of a level
+// (the four children's span slots, their aggregates) are made unconditional at
+// a clamped index and issued together: a conditional load would be waited for
+// inside its branch, one child after the other.
 // ----------------------------------------------------------------------------
 constexpr uint32_t kTopThreads = 256;
-constexpr uint32_t kTopLdsCells = 256;            // spanning cells aggregated in LDS
-constexpr uint32_t kTopLds = 5 * kTopLdsCells;    // their slots, then preloaded children
+constexpr uint32_t kTopLdsCells = 256;      // spanning cells aggregated in LDS (sets of several simulations)
+constexpr uint32_t kTopLdsCellsOne = 768;   // the same for a set of one simulation (nothing runs beside it)
+constexpr uint32_t kSpanNodeMask = (1u << 26) - 1u;  // span entry: (cell, depth << 26 | node)
 
-// Aggregate of spanning cell idx (depth d) from its children in global memory;
-// writes its f64 aggregate, its cell's f32 aggregates and its node's.
-__device__ __forceinline__ Agg agg_span_global(const SimDesc& S, uint32_t idx) {
-    const uint4 ch = S.cchild[idx];
+// Sum of present children in quadrant order (c[q] == kNoChild: absent).
+__device__ __forceinline__ Agg sum_children(const uint32_t c[4], const double m[4], const double x[4],
+                                            const double y[4]) {
     double mass = 0.0, wx = 0.0, wy = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint32_t c = (&ch.x)[q];
-        if (c == kNoChild) continue;
-        const Agg a = S.cagg[c];
-        mass = mass + a.m;
-        wx = wx + a.m * a.x;
-        wy = wy + a.m * a.y;
+        if (c[q] == kNoChild) continue;
+        mass = mass + m[q];
+        wx = wx + m[q] * x[q];
+        wy = wy + m[q] * y[q];
     }
     return Agg{mass, wx / mass, wy / mass, 0.0};
+}
+
+// Aggregate of spanning cell idx from its children's aggregates in global memory.
+__device__ __forceinline__ Agg agg_span_global(const SimDesc& S, uint32_t idx) {
+    const uint4 ch = S.cchild[idx];
+    uint32_t c[4];
+    double m[4], x[4], y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        c[q] = (&ch.x)[q];
+        const Agg a = S.cagg[c[q] == kNoChild ? idx : c[q]];
+        m[q] = a.m;
+        x[q] = a.x;
+        y[q] = a.y;
+    }
+    return sum_children(c, m, x, y);
 }
 
 __device__ __forceinline__ void store_span(const SimDesc& S, uint32_t idx, uint32_t node, const Agg& a) {
@@ -1882,14 +1935,18 @@ __device__ __forceinline__ void store_span(const SimDesc& S, uint32_t idx, uint3
 // Also gathers the sticky error words as k_collect_errors does (errs[0] = the
 // set's, errs[1 + s] = simulation s's): it runs last in a build, and no kernel
 // after it in a step sets them.
+template <uint32_t kTopLdsCells>
 __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims,
                                                          const uint32_t* __restrict__ ctl,
                                                          uint32_t* __restrict__ errs) {
+    constexpr uint32_t kTopLds = 5 * kTopLdsCells;  // the cells' slots, then preloaded children
     __shared__ double s_m[kTopLds], s_x[kTopLds], s_y[kTopLds];
-    __shared__ uint32_t s_ref[kTopLdsCells][4];
+    __shared__ uint4 s_ref[kTopLdsCells];
     __shared__ uint32_t s_idx[kTopLdsCells], s_node[kTopLdsCells];  // cell and node index
+    __shared__ uint16_t s_list[kTopLdsCells];                        // slots bucketed by depth
     __shared__ uint8_t s_depth[kTopLdsCells];
-    __shared__ uint32_t s_next, s_dmask, s_cnt[kLevels], s_off[kLevels + 1];
+    __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
+    static_assert(kTopLdsCells <= 65536, "LDS slots are 16-bit");
     const SimDesc& S = sim_at(sims, blockIdx.x);
     if (threadIdx.x == 0) {
         errs[1 + blockIdx.x] = S.meta->err;
@@ -1897,41 +1954,37 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     }
     const uint32_t nspan = min(S.meta->nspan, S.span_cap);
     if (nspan == 0) return;
-    constexpr uint32_t kNodeMask = (1u << 26) - 1u;
-    if (threadIdx.x == 0) s_dmask = 0u;
+    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) s_next = nspan;
+    __syncthreads();
     if (nspan <= kTopLdsCells) {
-        if (threadIdx.x == 0) s_next = nspan;
-        __syncthreads();
         for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-            // three dependent gathers: the span entry, the cell's children, the
-            // children's span slots (then their aggregates), each level's loads
-            // issued together
+            // three levels of gathers: the span entry, the cell's children, then
+            // the children's span slots and aggregates
             const uint2 se = S.span[i];
-            const uint32_t idx = se.x, depth = se.y >> 26;
+            const uint32_t idx = se.x, depth = min(se.y >> 26, kLevels - 1u);
             s_idx[i] = idx;
+            s_node[i] = se.y & kSpanNodeMask;
             s_depth[i] = (uint8_t)depth;
-            s_node[i] = se.y & kNodeMask;
-            atomicOr(&s_dmask, 1u << depth);
+            atomicAdd(&s_cnt[depth], 1u);
             const uint4 ch = S.cchild[idx];
-            uint32_t slot[4];
+            uint32_t c[4], info[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t c = (&ch.x)[q];
-                slot[q] = c != kNoChild ? S.cells[c].info >> kInfoSpanShift : 0u;
+                c[q] = (&ch.x)[q];
+                info[q] = S.cells[c[q] == kNoChild ? idx : c[q]].info;
             }
             Agg ca[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t c = (&ch.x)[q];
-                ca[q] = c != kNoChild && slot[q] == 0u ? S.cagg[c] : Agg{0.0, 0.0, 0.0, 0.0};
-            }
+            for (int q = 0; q < 4; ++q) ca[q] = S.cagg[c[q] == kNoChild ? idx : c[q]];
+            uint4 ref;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t c = (&ch.x)[q];
-                const bool preload = c != kNoChild && slot[q] == 0u;
+                const uint32_t slot = c[q] != kNoChild ? info[q] >> kInfoSpanShift : 0u;
+                const bool preload = c[q] != kNoChild && slot == 0u;
                 // preload slots: one LDS atomic per wavefront, ranks from the ballot
                 const unsigned long long m = __ballot(preload);
-                uint32_t r = c == kNoChild ? kNoChild : slot[q] - 1u;
+                uint32_t r = c[q] == kNoChild ? kNoChild : slot - 1u;
                 if (m) {
                     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
                     uint32_t first = 0u;
@@ -1944,26 +1997,43 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
                         s_y[r] = ca[q].y;
                     }
                 }
-                s_ref[i][q] = r;
+                (&ref.x)[q] = r;
             }
+            s_ref[i] = ref;
         }
         __syncthreads();
-        const uint32_t dmask = s_dmask;
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (uint32_t d = 0; d < kLevels; ++d) {
+                s_off[d] = acc;
+                acc += s_cnt[d];
+                s_cnt[d] = 0u;
+            }
+            s_off[kLevels] = acc;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+            const uint32_t d = s_depth[i];
+            s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = (uint16_t)i;
+        }
+        __syncthreads();
         for (int d = 15; d >= 0; --d) {
-            if (!((dmask >> d) & 1u)) continue;  // (uniform) no spanning cell at this depth
-            for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-                if (s_depth[i] != (uint32_t)d) continue;
-                double mass = 0.0, wx = 0.0, wy = 0.0;
+            const uint32_t lo = s_off[d], hi = s_off[d + 1];
+            if (lo == hi) continue;  // (uniform) no spanning cell at this depth
+            for (uint32_t k = lo + threadIdx.x; k < hi; k += kTopThreads) {
+                const uint32_t i = s_list[k];
+                const uint4 ref = s_ref[i];
+                uint32_t c[4];
+                double m[4], x[4], y[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const uint32_t r = s_ref[i][q];
-                    if (r == kNoChild) continue;
-                    const double cm = s_m[r];
-                    mass = mass + cm;
-                    wx = wx + cm * s_x[r];
-                    wy = wy + cm * s_y[r];
+                    c[q] = (&ref.x)[q];
+                    const uint32_t r = c[q] == kNoChild ? i : c[q];
+                    m[q] = s_m[r];
+                    x[q] = s_x[r];
+                    y[q] = s_y[r];
                 }
-                const Agg a{mass, wx / mass, wy / mass, 0.0};
+                const Agg a = sum_children(c, m, x, y);
                 s_m[i] = a.m;
                 s_x[i] = a.x;
                 s_y[i] = a.y;
@@ -1974,9 +2044,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         }
         return;
     }
-    // listed mode: counting sort of the span slots by depth into spanbuf
-    if (threadIdx.x < kLevels) s_cnt[threadIdx.x] = 0u;
-    __syncthreads();
+    // listed mode: counting sort of the span entries by depth into spanbuf
     for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
         atomicAdd(&s_cnt[min(S.span[i].y >> 26, kLevels - 1u)], 1u);
     __syncthreads();
@@ -1991,15 +2059,16 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
-        const uint32_t d = min(S.span[i].y >> 26, kLevels - 1u);
-        S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = i;
+        const uint2 se = S.span[i];
+        const uint32_t d = min(se.y >> 26, kLevels - 1u);
+        S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = se;
     }
     __syncthreads();  // (also orders the list's global stores before the rounds' loads)
     for (int d = 15; d >= 0; --d) {
         if (s_off[d] == s_off[d + 1]) continue;  // (uniform) no spanning cell at this depth
         for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kTopThreads) {
-            const uint2 se = S.span[S.spanbuf[k]];
-            store_span(S, se.x, se.y & kNodeMask, agg_span_global(S, se.x));
+            const uint2 se = S.spanbuf[k];
+            store_span(S, se.x, se.y & kSpanNodeMask, agg_span_global(S, se.x));
         }
         __syncthreads();  // the next depth reads this depth's global stores
     }

@@ -115,7 +115,7 @@ size_t carve(SimDesc& d, char* base, uint32_t n, bool tree) {
     d.cchild = (uint4*)take(16ull * cap);
     d.cagg = (Agg*)take(sizeof(Agg) * cap);
     d.span = (uint2*)take(8ull * scap);
-    d.spanbuf = (uint32_t*)take(4ull * scap);
+    d.spanbuf = (uint2*)take(8ull * scap);
     d.nodes = (NodeHot*)take(sizeof(NodeHot) * (cap + n));
     d.ncold = (NodeCold*)take(sizeof(NodeCold) * (cap + n));
     d.nbase = (uint32_t*)take(4ull * n);
@@ -378,8 +378,13 @@ struct SimSet {
     }
     void agg(hipStream_t s) {
         if (single_topo()) return;  // done by k_topo_single
-        hipLaunchKernelGGL(k_agg_top, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s, d_desc, d_ctl,
-                           d_errs);
+        // a set of one simulation stages more spanning cells in LDS (the small
+        // footprint matters only beside another group's kernels)
+        if (desc.size() == 1)
+            hipLaunchKernelGGL(k_agg_top<kTopLdsCellsOne>, dim3(1), dim3(kTopThreads), 0, s, d_desc, d_ctl, d_errs);
+        else
+            hipLaunchKernelGGL(k_agg_top<kTopLdsCells>, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s,
+                               d_desc, d_ctl, d_errs);
         HIP_OK(hipGetLastError());
     }
     void build(hipStream_t s) {
@@ -1375,6 +1380,30 @@ int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]) {
         out[0] = kin + potential;
         out[1] = lz;
         out[2] = kin;
+        return 0;
+    });
+}
+
+int gbh_check_thresholds(gbh_device* dev, const float* half, const float* theta, uint32_t count, float* out) {
+    return guarded([&] {
+        if (!dev || (count && (!half || !theta || !out))) {
+            set_error("null argument");
+            return -1;
+        }
+        if (count == 0) return 0;
+        hipStream_t s = dev->stream;
+        float* d = nullptr;
+        HIP_OK(hipMalloc((void**)&d, 12ull * count));
+        HIP_OK(hipMemcpyAsync(d, half, 4ull * count, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(d + count, theta, 4ull * count, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_check_thresholds, dim3(div_up(count, 256u)), dim3(256), 0, s, d, d + count, count,
+                           d + 2ull * count);
+        const hipError_t e = hipGetLastError();
+        if (e == hipSuccess) (void)hipMemcpyAsync(out, d + 2ull * count, 4ull * count, hipMemcpyDeviceToHost, s);
+        const hipError_t e2 = hipStreamSynchronize(s);
+        (void)hipFree(d);
+        HIP_OK(e);
+        HIP_OK(e2);
         return 0;
     });
 }

@@ -355,6 +355,11 @@ int gbh_diag_energy(gbh_state* st, double softening, double g, double out[3]);
  * guard-free interaction over every f32 in [2^-126, 2^126); first_bad = smallest
  * failing pattern of each. */
 int gbh_check_rounding(gbh_device* dev, uint64_t mismatches[3], uint32_t first_bad[3]);
+/* Device check (no reference counterpart) of the per-depth opening thresholds:
+ * out[i] = the smallest positive f32 d2 with fl(2 half[i] / fl(sqrt(d2))) <
+ * theta[i] as the builds compute it, NaN when none is (theta <= 0), so that
+ * d2 >= out[i] <=> the criterion of nbody.wgsl:105-108 holds. */
+int gbh_check_thresholds(gbh_device* dev, const float* half, const float* theta, uint32_t count, float* out);
 
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel
  * (k_copy_calibration); GPU time in *ms.  Used to calibrate hardware traffic
