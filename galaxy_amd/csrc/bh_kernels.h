@@ -1547,16 +1547,34 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         // ends, then their skip targets and thresholds, then stores.  (The
         // compiler cannot move a load across a store to another array, so a
         // load-store loop would pay two memory latencies per cell.)
+        // (Loads at clamped indices, unconditional: a conditional load would be
+        // waited for inside its branch, one cell after the other.)
         for (uint32_t k0 = 0; k0 < count; k0 += kLinkBatch) {
             uint32_t ev[kLinkBatch], sk[kLinkBatch];
             float tv[kLinkBatch];
 #pragma unroll
-            for (uint32_t j = 0; j < kLinkBatch; ++j) ev[j] = k0 + j + 1u < count ? S.cend[base + k0 + j] : 0u;
+            for (uint32_t j = 0; j < kLinkBatch; ++j) {
+                const uint32_t e = S.cend[base + min(k0 + j, count - 1u)];
+                ev[j] = k0 + j + 1u < count ? e : 0u;
+            }
+            uint32_t nbe[kLinkBatch];
+            float tr[kLinkBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kLinkBatch; ++j) {
+                nbe[j] = S.nbase[min(ev[j], n - 1u)];
+                tr[j] = S.meta->T[min(dmin + k0 + j, kLevels - 1u)];
+            }
+            // The values are consumed here, all at once: otherwise the compiler
+            // moves each load under the condition that selects it and waits for
+            // it there, one cell after the other.
+            static_assert(kLinkBatch == 4, "the register list below names four cells");
+            asm volatile("" : "+v"(nbe[0]), "+v"(nbe[1]), "+v"(nbe[2]), "+v"(nbe[3]), "+v"(tr[0]), "+v"(tr[1]),
+                         "+v"(tr[2]), "+v"(tr[3]));
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
                 const bool internal = k0 + j + 1u < count;
-                sk[j] = internal ? (ev[j] < n ? S.nbase[ev[j]] : nnodes) : 0u;
-                tv[j] = internal ? S.meta->T[dmin + k0 + j] : 0.0f;
+                sk[j] = internal ? (ev[j] < n ? nbe[j] : nnodes) : 0u;
+                tv[j] = internal ? tr[j] : 0.0f;
             }
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
