@@ -8,7 +8,7 @@ The search space:
                          the reference default 4 (nbody/src/lib.rs:9).
   sort_tile              keys per radix-sort workgroup: 2048, 1024
   traverse_block         traversal workgroup size: 256, 128, 64 threads
-  traverse_waves_per_cu  persistent traversal wavefronts per CU: 32, 24, 20, 16
+  traverse_waves_per_cu  persistent traversal wavefronts per CU: 32, 24, 20, 16, 12
   streams                groups of simulations stepped on streams of their own, so
                          their kernels overlap: 1, 2
 
@@ -37,7 +37,7 @@ SPACE = {
     "bucket": (4, 8, 16),
     "sort_tile": (2048, 1024),
     "traverse_block": (256, 128, 64),
-    "traverse_waves_per_cu": (32, 24, 20, 16),
+    "traverse_waves_per_cu": (32, 24, 20, 16, 12),
     "streams": (1, 2),
 }
 

@@ -24,7 +24,7 @@ constexpr uint32_t kTile = 256;          // bodies per tile: bounds, KDK
 constexpr uint32_t kTopoTile = 1024;     // sorted positions per topology workgroup
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kRadix = 256;
-constexpr uint32_t kWaveTargets = 64;    // traversal targets per wavefront
+constexpr uint32_t kMaxLaneTargets = 2;  // traversal targets per lane (work items of 64 or 128 targets)
 constexpr uint32_t kLevels = 17;         // Morton depths 0..16
 constexpr uint32_t kInfoLeaf = 1u << 8;
 constexpr uint32_t kInfoSpanShift = 9;   // (span slot + 1) << 9 for internal cells that cross tiles
@@ -2213,18 +2213,15 @@ __device__ __forceinline__ F4v load_node(__amdgpu_buffer_rsrc_t r, uint32_t inde
     return struct_buffer_load_v4f32(r, (int)index, 0, 0, 0);
 }
 
-// kPow4: the thresholds scale by powers of four (SimMeta::pow4) and the words
-// have layout 1, so the opening test is one integer comparison against T[0]
-// (pow4_thresholds): bits(d2) + (w << 22) >= bits(T[0]), where w << 22 is the
-// code's exponent offset (node_word).  Otherwise the threshold comes from the
-// code table in LDS and the size from the layout's shift.
-template <bool kEvidence, bool kGeneral, bool kFast, bool kPow4 = false>
+// One target per lane.  The threshold comes from the code table in LDS (or,
+// in general mode, from the node's cold record) and the size from the word
+// layout's shift.
+template <bool kEvidence, bool kGeneral, bool kFast>
 struct NodeWalk {
     __amdgpu_buffer_rsrc_t nh;
     const __attribute__((address_space(1))) U4v* nc;
     uint32_t tab, nn, p, self;
     float tx, ty, eps2, G;
-    uint32_t t0_bits = 0;
     uint32_t wshift = kSizeShift;  // the word layout's size shift
     float ax = 0.f, ay = 0.f;
     WalkEvidence ev;
@@ -2238,7 +2235,7 @@ struct NodeWalk {
         const float dd = dx * dx;
         const float ee = dy * dy;
         const float d2 = dd + ee;
-        const uint32_t size_m1 = w >> (kPow4 ? kSizeShiftNarrow : wshift);  // subtree [c, c + size_m1]
+        const uint32_t size_m1 = w >> wshift;  // subtree [c, c + size_m1]
         bool inside;
         bool open_ok;  // d2 >= the node's threshold
         U4v l;
@@ -2246,9 +2243,6 @@ struct NodeWalk {
         if (kGeneral) {
             inside = (p - l.x) < l.y;
             open_ok = d2 >= __uint_as_float(l.z);
-        } else if (kPow4) {
-            inside = (self - c) <= size_m1;
-            open_ok = __float_as_uint(d2) + (w << 22) >= t0_bits;
         } else {
             inside = (self - c) <= size_m1;
             open_ok = d2 >= *(const __attribute__((address_space(3))) float*)(uintptr_t)((w & kCodeField) | tab);
@@ -2290,6 +2284,91 @@ struct NodeWalk {
         }
     }
 };
+
+// K targets per lane, sorted neighbours (positions K l .. K l + K - 1
+// of the item): their walks visit mostly the same nodes, so the lane walks the
+// union of them with one gather per node.  Each target keeps its own cursor,
+// the next node it has to look at; the lane's cursor is the smallest one, and a
+// node is evaluated for the targets whose cursor it is, so every target sees its
+// own nodes in its own (increasing) order and its sums equal its own walk's.
+// It serves simulations whose thresholds scale by powers of four (SimMeta::
+// pow4) and whose words have layout 1, so the opening test is one integer
+// comparison against T[0] (pow4_thresholds): bits(d2) + (w << 22) >= bits(T[0]),
+// where w << 22 is the code's exponent offset (node_word).  As in the fast
+// NodeWalk, the end test runs once per two steps (zero record).
+template <uint32_t K>
+struct MultiWalk {
+    __amdgpu_buffer_rsrc_t nh;
+    uint32_t nn, t0_bits;
+    float eps2, G;
+    uint32_t self[K], cur[K];  // cur: each target's cursor (nn: none)
+    float tx[K], ty[K];
+    float ax[K] = {}, ay[K] = {};
+
+    __device__ __forceinline__ uint32_t step(uint32_t c, const F4v& g, F4v& gn) {
+        const uint32_t w = __float_as_uint(g.w);
+        const uint32_t size_m1 = w >> kSizeShiftNarrow;
+        const uint32_t skip = c + 1u + size_m1;
+        float dx[K], dy[K], d2[K];
+        bool acc[K];
+        uint32_t next = 0xffffffffu;
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            dx[k] = g.x - tx[k];
+            dy[k] = g.y - ty[k];
+            const float dd = dx[k] * dx[k];
+            const float ee = dy[k] * dy[k];
+            d2[k] = dd + ee;
+            const bool at = K == 1 || cur[k] == c;
+            acc[k] = at && (self[k] - c) > size_m1 && __float_as_uint(d2[k]) + (w << 22) >= t0_bits;
+            if (at) cur[k] = acc[k] ? skip : c + 1u;
+            next = min(next, cur[k]);  // greater than c
+        }
+        gn = load_node(nh, next);
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            if (acc[k]) point_mass_pos(dx[k], dy[k], d2[k], g.z, eps2, G, ax[k], ay[k]);
+        return next;
+    }
+
+    __device__ __forceinline__ void run() {
+        uint32_t c = cur[0];
+#pragma unroll
+        for (uint32_t k = 1; k < K; ++k) c = min(c, cur[k]);
+        F4v g0 = load_node(nh, c), g1;
+        while (c < nn) {
+            c = step(c, g0, g1);
+            c = step(c, g1, g0);
+        }
+    }
+};
+
+// The multi-target walk of one work item (targets base .. base + 64 K - 1).
+template <uint32_t K>
+__device__ __forceinline__ void multi_walk(const SimDesc& S, uint32_t base, uint32_t n, __amdgpu_buffer_rsrc_t nh,
+                                           uint32_t nn, uint32_t t0_bits, float eps2, float G) {
+    MultiWalk<K> wk;
+    wk.nh = nh;
+    wk.nn = nn;
+    wk.t0_bits = t0_bits;
+    wk.eps2 = eps2;
+    wk.G = G;
+    const uint32_t p0 = base + K * lane_id();
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t q = min(p0 + k, n - 1u);
+        const float4 b = S.sbody[q];
+        wk.tx[k] = b.x;
+        wk.ty[k] = b.y;
+        wk.self[k] = S.bnode[q];
+        wk.cur[k] = p0 + k < n ? 0u : nn;
+    }
+    wk.run();
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k)
+        if (p0 + k < n) S.acc[S.val0[p0 + k]] = make_float2(wk.ax[k], wk.ay[k]);
+}
 
 // Traversal work items by XCD (mode 0).  For a set of many simulations the host
 // lays the items out as one contiguous, balanced range per XCD (simulations
@@ -2335,13 +2414,16 @@ __device__ uint32_t claim_item(uint32_t* ctl, uint32_t mode, uint32_t& q, uint32
     }
 }
 
+// Work items of 64 lane_targets targets (lane_targets 1 or 2, the set's
+// choice): the multi-target walk where it applies, else lane_targets parts of
+// 64 targets one after the other, one target per lane.
 template <bool kEvidence>
 __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restrict__ sims,
                                                         const TileRef* __restrict__ wtiles,
                                                         uint32_t nwtiles, uint32_t* __restrict__ ctl,
                                                         uint32_t* __restrict__ ev_stats,
                                                         unsigned long long* __restrict__ ev_hash,
-                                                        uint32_t ev_sim) {
+                                                        uint32_t ev_sim, uint32_t lane_targets) {
     __shared__ __attribute__((aligned(64))) float s_tab[256 / 64][kCodes];
     float* tabp = s_tab[threadIdx.x >> 6];
     const uint32_t tab = __builtin_amdgcn_readfirstlane(
@@ -2366,8 +2448,6 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const TileRef tr = wtiles[item];
         const SimDesc& S = sim_at(sims, tr.sim);
         const uint32_t n = S.n;
-        const uint32_t p = tr.base + lane;
-        const bool valid = p < n;
         // a failed build (its error bits are reported by the host) cannot send
         // the walk past the node arrays
         const uint32_t nn = __builtin_amdgcn_readfirstlane(min(S.meta->nnodes, S.node_cap - 1u));
@@ -2381,50 +2461,56 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const float eps2 = eps * eps;
         const float G = S.G;
         const bool fast = geom_ok != 0u && eps2 > 0.0f && eps2 <= 3.4028234663852886e38f;
-        // this wavefront's copy of the code table (LDS is in order per wavefront)
-        if (lane < kCodes) tabp[lane] = S.meta->tab[lane];
-        __builtin_amdgcn_wave_barrier();
-        float tx = 0.f, ty = 0.f;
-        uint32_t self = 0;
-        if (valid) {
-            const float4 b = S.sbody[p];
-            tx = b.x;
-            ty = b.y;
-            if (!general) self = S.bnode[p];
-        }
-        float ax = 0.f, ay = 0.f;
-        WalkEvidence ev;
-        const uint32_t c0 = valid ? 0u : nn;
         const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, nn);
-        const GlobalU4 nc = (GlobalU4)(const void*)S.ncold;
-        if (general) {
-            NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
-            wk.run(c0);
-            ax = wk.ax, ay = wk.ay, ev = wk.ev;
-        } else if (fast && pow4 && narrow != 0u && !kEvidence) {
-            NodeWalk<kEvidence, false, true, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, t0_bits};
-            wk.run(c0);
-            ax = wk.ax, ay = wk.ay, ev = wk.ev;
-        } else if (fast) {
-            NodeWalk<kEvidence, false, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
-            wk.run(c0);
-            ax = wk.ax, ay = wk.ay, ev = wk.ev;
+        if (!kEvidence && !general && fast && pow4 && narrow != 0u) {
+            if (lane_targets == 2u) multi_walk<2>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+            else multi_walk<1>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
         } else {
-            NodeWalk<kEvidence, false, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, 0u, wshift};
-            wk.run(c0);
-            ax = wk.ax, ay = wk.ay, ev = wk.ev;
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            const uint32_t body = S.val0[p];
-            S.acc[body] = make_float2(ax, ay);
-            if (kEvidence && tr.sim == ev_sim) {
-                if (ev_stats) {
-                    ev_stats[3 * (size_t)body + 0] = ev.visited;
-                    ev_stats[3 * (size_t)body + 1] = ev.accepted;
-                    ev_stats[3 * (size_t)body + 2] = ev.direct;
+            // this wavefront's copy of the code table (LDS is in order per wavefront)
+            if (lane < kCodes) tabp[lane] = S.meta->tab[lane];
+            __builtin_amdgcn_wave_barrier();
+            const GlobalU4 nc = (GlobalU4)(const void*)S.ncold;
+            // the item's 64-target parts, one target per lane
+            for (uint32_t half = 0; half < lane_targets; ++half) {
+                const uint32_t p = tr.base + 64u * half + lane;
+                const bool valid = p < n;
+                float tx = 0.f, ty = 0.f;
+                uint32_t self = 0;
+                if (valid) {
+                    const float4 b = S.sbody[p];
+                    tx = b.x;
+                    ty = b.y;
+                    if (!general) self = S.bnode[p];
                 }
-                if (ev_hash) ev_hash[body] = ev.h;
+                float ax = 0.f, ay = 0.f;
+                WalkEvidence ev;
+                const uint32_t c0 = valid ? 0u : nn;
+                if (general) {
+                    NodeWalk<kEvidence, true, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, wshift};
+                    wk.run(c0);
+                    ax = wk.ax, ay = wk.ay, ev = wk.ev;
+                } else if (fast) {
+                    NodeWalk<kEvidence, false, true> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, wshift};
+                    wk.run(c0);
+                    ax = wk.ax, ay = wk.ay, ev = wk.ev;
+                } else {
+                    NodeWalk<kEvidence, false, false> wk{nh, nc, tab, nn, p, self, tx, ty, eps2, G, wshift};
+                    wk.run(c0);
+                    ax = wk.ax, ay = wk.ay, ev = wk.ev;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (valid) {
+                    const uint32_t body = S.val0[p];
+                    S.acc[body] = make_float2(ax, ay);
+                    if (kEvidence && tr.sim == ev_sim) {
+                        if (ev_stats) {
+                            ev_stats[3 * (size_t)body + 0] = ev.visited;
+                            ev_stats[3 * (size_t)body + 1] = ev.accepted;
+                            ev_stats[3 * (size_t)body + 2] = ev.direct;
+                        }
+                        if (ev_hash) ev_hash[body] = ev.h;
+                    }
+                }
             }
         }
         // the next item is claimed when this one is done (a claim made earlier

@@ -190,6 +190,7 @@ struct SimSet {
     uint32_t trav_block = 256;        // traversal workgroup size (64, 128 or 256)
     uint32_t trav_waves_per_cu = 32;  // persistent traversal wavefronts per CU
     uint32_t traverse_blocks = 0;
+    uint32_t lane_targets = 1;        // traversal targets per lane (build_tables)
 
     ~SimSet() { release(); }
 
@@ -244,13 +245,23 @@ struct SimSet {
         // (a simulation larger than one XCD's share): one range in size order.
         // Within a range, larger simulations come first (longer walks) for a
         // short tail.
+        // Targets per lane: two (items of 128 targets, the multi-target walk)
+        // when the set has items enough to keep the persistent grid busy twice
+        // over; else one, so that a small set still spreads over the device.
+        {
+            uint64_t items2 = 0;
+            for (const SimDesc& d : desc) items2 += div_up(d.n, 2u * 64u);
+            const uint64_t grid_waves = (uint64_t)(dev ? dev->cu_count : 256) * trav_waves_per_cu;
+            lane_targets = items2 >= 2ull * grid_waves ? 2u : 1u;
+        }
+        const uint32_t per_item = 64u * lane_targets;
         std::vector<uint32_t> order(desc.size());
         for (uint32_t s = 0; s < order.size(); ++s) order[s] = s;
         std::stable_sort(order.begin(), order.end(),
                          [&](uint32_t a, uint32_t b) { return desc[a].n > desc[b].n; });
         uint64_t total = 0, largest = 0;
         for (const SimDesc& d : desc) {
-            const uint64_t items = div_up(d.n, kWaveTargets);
+            const uint64_t items = div_up(d.n, per_item);
             total += items;
             largest = std::max(largest, items);
         }
@@ -258,10 +269,10 @@ struct SimSet {
         if (mode == 0u) {
             for (uint32_t x = 0; x < kXcds; ++x)
                 for (size_t k = x; k < order.size(); k += kXcds)
-                    for (uint32_t b = 0; b < desc[order[k]].n; b += kWaveTargets) twave.push_back({order[k], b});
+                    for (uint32_t b = 0; b < desc[order[k]].n; b += per_item) twave.push_back({order[k], b});
         } else {
             for (uint32_t s : order)
-                for (uint32_t b = 0; b < desc[s].n; b += kWaveTargets) twave.push_back({s, b});
+                for (uint32_t b = 0; b < desc[s].n; b += per_item) twave.push_back({s, b});
         }
         const uint32_t nt = (uint32_t)twave.size(), q = nt / kXcds, r = nt % kXcds;
         for (uint32_t x = 0; x < kXcds; ++x) {
@@ -414,11 +425,12 @@ struct SimSet {
         }
         if (ev_stats || ev_hash)
             hipLaunchKernelGGL(k_traverse_nodes<true>, dim3(traverse_blocks), dim3(trav_block), 0, s,
-                               d_desc, d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, ev_sim);
+                               d_desc, d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash, ev_sim,
+                               lane_targets);
         else
             hipLaunchKernelGGL(k_traverse_nodes<false>, dim3(traverse_blocks), dim3(trav_block), 0, s,
                                d_desc, d_twave, (uint32_t)twave.size(), d_ctl, ev_stats, ev_hash,
-                               0xffffffffu);
+                               0xffffffffu, lane_targets);
         HIP_OK(hipGetLastError());
     }
     // body-node cold records for an evidence traversal (the step leaves them unwritten)

@@ -25,7 +25,7 @@ def synthetic_cost(choice):
         c += {256: 0.05, 128: 0.0, 64: 0.08}[choice.traverse_block]
     else:
         c += {256: 0.0, 128: 0.04, 64: 0.08}[choice.traverse_block]
-    c += {32: 0.2, 24: 0.1, 20: 0.06, 16: 0.03}[choice.traverse_waves_per_cu]
+    c += {32: 0.2, 24: 0.1, 20: 0.06, 16: 0.03, 12: 0.05}[choice.traverse_waves_per_cu]
     c -= 0.05 if choice.streams == 2 else 0.0
     return c
 
