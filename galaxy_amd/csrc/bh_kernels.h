@@ -1257,10 +1257,16 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     tb.build(base, n, s_dmin);
     auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
     // columns: existence from the parent group's size, leaf depth from the key at p + bucket
+    // The first kLeafBatch bodies of each column's leaf are loaded here, so the
+    // look-back below hides their latency (loaded later, behind the column's
+    // stores, they would wait for those stores: one counter orders both).
+    float4 lead[kTopoPer][kLeafBatch];
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t o = j * kTopoThreads + threadIdx.x;
         const uint32_t p = base + o;
+#pragma unroll
+        for (uint32_t k = 0; k < kLeafBatch; ++k) lead[j][k] = S.sbody[min(p + k, n - 1u)];
         uint32_t count = 0, pl = 0;
         const uint32_t dmin = s_dmin[o];
         if (p < n && dmin <= maxd) {
@@ -1357,17 +1363,19 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
             if (leaf) {
                 double mass = 0.0, wx = 0.0, wy = 0.0;
                 const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
-                // kLeafBatch bodies at a time: loads first, then the stores
+                // kLeafBatch bodies at a time: loads first, then the stores (the
+                // first batch was loaded before the look-back)
                 for (uint32_t q0 = p; q0 < e; q0 += kLeafBatch) {
                     float4 bv[kLeafBatch];
 #pragma unroll
-                    for (uint32_t j = 0; j < kLeafBatch; ++j)
-                        bv[j] = q0 + j < e ? S.sbody[q0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    for (uint32_t k = 0; k < kLeafBatch; ++k)
+                        bv[k] = q0 == p ? lead[j][k]
+                                        : (q0 + k < e ? S.sbody[q0 + k] : make_float4(0.f, 0.f, 0.f, 0.f));
 #pragma unroll
-                    for (uint32_t j = 0; j < kLeafBatch; ++j) {
-                        const uint32_t q = q0 + j;
+                    for (uint32_t k = 0; k < kLeafBatch; ++k) {
+                        const uint32_t q = q0 + k;
                         if (q >= e) break;
-                        const float4 bq = bv[j];
+                        const float4 bq = bv[k];
                         const double bm = bq.z;
                         mass = mass + bm;
                         wx = wx + bm * (double)bq.x;
