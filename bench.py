@@ -165,17 +165,26 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
     b.close()
     one_group = None
     if choice.streams != 1:
-        # the same shape as one group of 256 simulations on one stream: one launch per
-        # phase (the tuned headline steps two groups, each with its own launches)
-        b1 = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=choice.sort_tile,
-                   traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu,
+        # One group of 256 simulations on one stream: one launch per phase (the tuned
+        # headline steps two groups, each with its own launches), at the best one-group
+        # shape the tuner measured (else the headline's shape with one stream).  Its
+        # phase timing gives the traversal kernel alone, one launch of the whole set.
+        one = replace(choice, streams=1)
+        if tuning is not None:
+            ones = [(t, c) for c, t in tuning["measured"] if c["streams"] == 1 and c["bucket"] == choice.bucket]
+            if ones:
+                one = autotune.Choice(**min(ones, key=lambda x: x[0])[1])
+        b1 = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=one.sort_tile,
+                   traverse_block=one.traverse_block, traverse_waves_per_cu=one.traverse_waves_per_cu,
                    streams=1)
         b1.step(max(warmup, 1))
         t1 = time.perf_counter()
         b1.step(steps)
         el1 = time.perf_counter() - t1
+        ph1 = [b1.phase_ms() for _ in range(3)]
         b1.close()
-        one_group = dict(ms_per_step=1e3 * el1 / steps, body_steps_per_s=total * steps / el1)
+        one_group = dict(choice=asdict(one), ms_per_step=1e3 * el1 / steps, body_steps_per_s=total * steps / el1,
+                         phase_ms={k: float(np.mean([p[k] for p in ph1])) for k in ph1[0]})
     tune_out = None
     if tuning is not None:
         tune_out = {"choice": asdict(choice), "cached": tuning["cached"],
@@ -443,6 +452,13 @@ def main():
     plain_trav = FLOPS_PER_INTERACTION * plain["interactions"] / (plain["traversal_ms"] * 1e-3) / 1e12
     model = bh_model_v1(batched["bodies"], batched["cells"], batched["interactions"], batched["ms_per_step"],
                         batched["phase_ms"])
+    one_launch = None
+    if batched["one_group"]:
+        # the kernel alone: one traversal launch over all 256 simulations (one group)
+        ms1 = batched["one_group"]["phase_ms"]["traversal"]
+        a1 = FLOPS_PER_INTERACTION * batched["interactions"] / (ms1 * 1e-3) / 1e12
+        one_launch = dict(achieved=a1, frac=a1 / FP32_PEAK_TFLOPS, launch_ms=ms1,
+                          choice=batched["one_group"]["choice"])
     line = {
         "metric": "body-steps/s batched+served vs plain at N=65,536; p50/p99 request latency",
         "value": batched["body_steps_per_s"],
@@ -498,6 +514,7 @@ def main():
             # kernel_ms_per_step sums the groups' launches, timed one after the other
             "launches_per_step": int(batched["choice"].get("streams", 1)),
             "kernel_ms_per_step": trav_ms,
+            "one_launch": one_launch,
             "plain_traversal_tflops": plain_trav,
             "plain_traversal_ms": plain["traversal_ms"],
             "step_bh_model_v1": model,

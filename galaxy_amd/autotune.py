@@ -15,13 +15,15 @@ The search space:
 The launch-shape choices leave every result bit-identical; the leaf size defines
 the tree, and the CPU restatement reproduces the tree and forces for any leaf size.
 Candidates are timed on the workload itself (whole HIP-graph steps after a
-warm-up) by coordinate descent from the defaults (passes repeated while one
-improves, at most three), and the winner is cached per
+warm-up) by coordinate descent from the defaults (traversal waves and groups as
+one joint coordinate; passes repeated while one improves, at most three), and
+the winner is cached per
 workload signature (sizes histogram, softening, theta, device) in a JSON file,
 so later runs of the same ensemble shape reuse it without measuring.
 """
 from __future__ import annotations
 
+import itertools
 import json
 import math
 import os
@@ -40,6 +42,11 @@ SPACE = {
     "traverse_waves_per_cu": (32, 24, 20, 16, 12),
     "streams": (1, 2),
 }
+
+# Coordinates of the descent.  Resident traversal waves and the number of
+# groups are searched together: two groups overlap well only with fewer
+# traversal waves per CU, so neither improves alone from the one-group optimum.
+COORDINATES = (("bucket",), ("sort_tile",), ("traverse_block",), ("traverse_waves_per_cu", "streams"))
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "galaxy_amd", "tune.json")
 
@@ -112,9 +119,9 @@ def tune(gpu: NbodyGpu, states, config: Config, dt: float, cache: str | None = D
     cost(best)
     for _ in range(3):  # coordinate descent, repeated while a pass improves (timings are kept)
         start = best
-        for name, values in SPACE.items():
-            for v in values:
-                cand = replace(best, **{name: v})
+        for names in COORDINATES:
+            for values in itertools.product(*(SPACE[n] for n in names)):
+                cand = replace(best, **dict(zip(names, values)))
                 if cost(cand) < cost(best):
                     best = cand
         if best == start:

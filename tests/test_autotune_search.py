@@ -15,9 +15,11 @@ class FakeGpu:
 
 
 def synthetic_cost(choice):
-    # best at (4, 2048, 256, 16, 2); block and waves interact: with 32 waves per CU
-    # the 128-thread block is better, so a single pass that tunes the block first
-    # (at the default 32 waves) settles on 128 and misses the optimum
+    # best at (4, 2048, 256, 16, 2).  Block and waves interact: with 32 waves per
+    # CU the 128-thread block is better, so a single pass that tunes the block first
+    # (at the default 32 waves) settles on 128 and misses the optimum.  Waves and
+    # groups interact too: one group prefers 32 waves, two groups only pay off at 16
+    # or fewer, so neither coordinate improves alone from (32, 1).
     c = 1.0
     c += {4: 0.0, 8: 0.3, 16: 0.9}[choice.bucket]
     c += {2048: 0.0, 1024: 0.02}[choice.sort_tile]
@@ -25,8 +27,10 @@ def synthetic_cost(choice):
         c += {256: 0.05, 128: 0.0, 64: 0.08}[choice.traverse_block]
     else:
         c += {256: 0.0, 128: 0.04, 64: 0.08}[choice.traverse_block]
-    c += {32: 0.2, 24: 0.1, 20: 0.06, 16: 0.03, 12: 0.05}[choice.traverse_waves_per_cu]
-    c -= 0.05 if choice.streams == 2 else 0.0
+    if choice.streams == 1:
+        c += {32: 0.0, 24: 0.05, 20: 0.08, 16: 0.1, 12: 0.15}[choice.traverse_waves_per_cu]
+    else:
+        c += {32: 0.3, 24: 0.1, 20: 0.0, 16: -0.1, 12: -0.05}[choice.traverse_waves_per_cu]
     return c
 
 
