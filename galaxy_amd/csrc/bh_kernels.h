@@ -2327,9 +2327,10 @@ struct MultiWalk {
             const float dd = dx[k] * dx[k];
             const float ee = dy[k] * dy[k];
             d2[k] = dd + ee;
+            // (bitwise, not short-circuit: straight-line selects, no branches)
             const bool at = K == 1 || cur[k] == c;
-            acc[k] = at && (self[k] - c) > size_m1 && __float_as_uint(d2[k]) + (w << 22) >= t0_bits;
-            if (at) cur[k] = acc[k] ? skip : c + 1u;
+            acc[k] = at & ((self[k] - c) > size_m1) & (__float_as_uint(d2[k]) + (w << 22) >= t0_bits);
+            cur[k] = acc[k] ? skip : (at ? c + 1u : cur[k]);
             next = min(next, cur[k]);  // greater than c
         }
         gn = load_node(nh, next);
