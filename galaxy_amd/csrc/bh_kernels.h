@@ -527,12 +527,14 @@ __device__ __forceinline__ void point_mass_d(float dx, float dy, float d2, float
 // The same where r2 = d2 + eps2 is finite and positive: then sqrt_rn takes its
 // finite_pos branch and s lies in [2^-75, 2^64], where rcp_rn_normal is the
 // correctly rounded reciprocal, so the result is identical.
+// kUnitG: G == 1 (then fl(G m) = m, and the product is left out).
+template <bool kUnitG = false>
 __device__ __forceinline__ void point_mass_pos(float dx, float dy, float d2, float m, float eps2,
                                                float G, float& ax, float& ay) {
     float r2 = d2 + eps2;
     float s = sqrt_rn_pos(r2);
     float inv_r = rcp_rn_normal(s);
-    float scale = G * m;
+    float scale = kUnitG ? m : G * m;
     scale = scale * inv_r;
     scale = scale * inv_r;
     scale = scale * inv_r;
@@ -2304,7 +2306,8 @@ struct NodeWalk {
 // comparison against T[0] (pow4_thresholds): bits(d2) + (w << 22) >= bits(T[0]),
 // where w << 22 is the code's exponent offset (node_word).  As in the fast
 // NodeWalk, the end test runs once per two steps (zero record).
-template <uint32_t K>
+// kUnitG: G == 1, where fl(G m) = m for every m, so the product is left out.
+template <uint32_t K, bool kUnitG>
 struct MultiWalk {
     __amdgpu_buffer_rsrc_t nh;
     uint32_t nn, t0_bits;
@@ -2313,10 +2316,22 @@ struct MultiWalk {
     float tx[K], ty[K];
     float ax[K] = {}, ay[K] = {};
 
+    // The opening test is evaluated as bits(d2) >= t0 - (w << 22) in signed 32-bit
+    // arithmetic (one subtraction per node, one comparison per target).  It is the
+    // same integer inequality: w << 22 is code << 24 (at most 15 << 24) plus 2^31
+    // for code 0, so without the flag t0 - (w << 22) lies in (-2^31, 2^31) and the
+    // unsigned sum bits(d2) + (w << 22) <= 0x7f800000 + (15 << 24) does not wrap;
+    // with it the difference wraps to t0 - 2^31 < 0 and both forms hold for every
+    // d2 >= +0 (bits(d2) < 2^31).  The cursor update uses that every cursor is at
+    // least c (c is their minimum): a target not at c keeps its cursor, which
+    // already exceeds c + 1.
     __device__ __forceinline__ uint32_t step(uint32_t c, const F4v& g, F4v& gn) {
         const uint32_t w = __float_as_uint(g.w);
         const uint32_t size_m1 = w >> kSizeShiftNarrow;
-        const uint32_t skip = c + 1u + size_m1;
+        const uint32_t c1 = c + 1u;
+        uint32_t skip = c1 + size_m1;
+        asm volatile("" : "+v"(skip));  // one sum for all targets: a select per target, not an add
+        const int tnode = (int)(t0_bits - (w << 22));
         float dx[K], dy[K], d2[K];
         bool acc[K];
         uint32_t next = 0xffffffffu;
@@ -2329,14 +2344,14 @@ struct MultiWalk {
             d2[k] = dd + ee;
             // (bitwise, not short-circuit: straight-line selects, no branches)
             const bool at = K == 1 || cur[k] == c;
-            acc[k] = at & ((self[k] - c) > size_m1) & (__float_as_uint(d2[k]) + (w << 22) >= t0_bits);
-            cur[k] = acc[k] ? skip : (at ? c + 1u : cur[k]);
+            acc[k] = at & ((self[k] - c) > size_m1) & ((int)__float_as_uint(d2[k]) >= tnode);
+            cur[k] = max(cur[k], acc[k] ? skip : c1);
             next = min(next, cur[k]);  // greater than c
         }
         gn = load_node(nh, next);
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k)
-            if (acc[k]) point_mass_pos(dx[k], dy[k], d2[k], g.z, eps2, G, ax[k], ay[k]);
+            if (acc[k]) point_mass_pos<kUnitG>(dx[k], dy[k], d2[k], g.z, eps2, G, ax[k], ay[k]);
         return next;
     }
 
@@ -2353,10 +2368,10 @@ struct MultiWalk {
 };
 
 // The multi-target walk of one work item (targets base .. base + 64 K - 1).
-template <uint32_t K>
+template <uint32_t K, bool kUnitG>
 __device__ __forceinline__ void multi_walk(const SimDesc& S, uint32_t base, uint32_t n, __amdgpu_buffer_rsrc_t nh,
                                            uint32_t nn, uint32_t t0_bits, float eps2, float G) {
-    MultiWalk<K> wk;
+    MultiWalk<K, kUnitG> wk;
     wk.nh = nh;
     wk.nn = nn;
     wk.t0_bits = t0_bits;
@@ -2472,8 +2487,13 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const bool fast = geom_ok != 0u && eps2 > 0.0f && eps2 <= 3.4028234663852886e38f;
         const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, nn);
         if (!kEvidence && !general && fast && pow4 && narrow != 0u) {
-            if (lane_targets == 2u) multi_walk<2>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
-            else multi_walk<1>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+            if (G == 1.0f) {
+                if (lane_targets == 2u) multi_walk<2, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                else multi_walk<1, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+            } else {
+                if (lane_targets == 2u) multi_walk<2, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                else multi_walk<1, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+            }
         } else {
             // this wavefront's copy of the code table (LDS is in order per wavefront)
             if (lane < kCodes) tabp[lane] = S.meta->tab[lane];
