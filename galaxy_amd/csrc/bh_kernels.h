@@ -21,7 +21,7 @@ namespace gbh {
 
 constexpr uint32_t kNoChild = 0xffffffffu;
 constexpr uint32_t kTile = 256;          // bodies per tile: bounds, KDK
-constexpr uint32_t kTopoTile = 1024;     // sorted positions per topology workgroup
+constexpr uint32_t kTopoTile = 1024;     // topology tile of a one-simulation set; limit of the one-workgroup builds
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxLaneTargets = 2;  // traversal targets per lane (work items of 64 or 128 targets)
@@ -992,9 +992,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_single(const SimDesc* __r
 // the keys: per-wavefront ballot masks of "dmin <= d" in LDS, per-wavefront
 // first/last boundaries within the tile, and per-tile summaries over the other
 // tiles of the simulation (k_topo_pre, k_topo_scan).
-constexpr uint32_t kTopoWaves = kTopoTile / 64;          // 64-position chunks of a tile
-constexpr uint32_t kTopoThreads = 1024;                   // threads per topology workgroup
-constexpr uint32_t kTopoPer = kTopoTile / kTopoThreads;   // positions per thread (strided)
+// The tiled kernels take the tile size kTT as a template argument: one position
+// per thread, kTT threads per workgroup.  Sets of several simulations use
+// kTopoTileMulti positions (workgroups small enough to run beside the other
+// group's traversal), sets of one simulation kTopoTile.
+constexpr uint32_t kTopoTileMulti = 512;
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
 constexpr uint32_t kLeafBatch = 4;           // leaf bodies whose loads are issued together
 constexpr uint32_t kNone = 0xffffffffu;
@@ -1011,9 +1013,11 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 // (x, y, m) of the body at sorted position p: its reads go to random body
 // indices (inside the last radix pass their latency held every tile's
 // workgroup), here they overlap the boundary work.
+template <uint32_t kTT>
 __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
                                               const TileRef* __restrict__ ttiles,
                                               uint32_t* __restrict__ tsum, uint32_t block, uint32_t grid) {
+    constexpr uint32_t kTopoThreads = kTT, kTopoPer = 1;
     // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
     __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
     const uint32_t t = xcd_tile(block, grid);
@@ -1073,10 +1077,11 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
     }
 }
 
-__global__ __launch_bounds__(kTopoThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
-                                                           const TileRef* __restrict__ ttiles,
-                                                           uint32_t* __restrict__ tsum) {
-    topo_pre_body(sims, ttiles, tsum, blockIdx.x, gridDim.x);
+template <uint32_t kTT>
+__global__ __launch_bounds__(kTT) void k_topo_pre(const SimDesc* __restrict__ sims,
+                                                  const TileRef* __restrict__ ttiles,
+                                                  uint32_t* __restrict__ tsum) {
+    topo_pre_body<kTT>(sims, ttiles, tsum, blockIdx.x, gridDim.x);
 }
 
 // Inclusive wavefront scan with max / min (all lanes active).
@@ -1102,14 +1107,14 @@ __device__ __forceinline__ uint32_t wave_incl_min_rev(uint32_t v) {  // min over
 // (psum[t][17+d], n: none).  Wavefront w handles depths w, w + 16, ...
 __global__ __launch_bounds__(64) void k_topo_scan(const SimDesc* __restrict__ sims,
                                                   const uint32_t* __restrict__ tsum,
-                                                  uint32_t* __restrict__ psum) {
+                                                  uint32_t* __restrict__ psum, uint32_t tile) {
     // One wavefront per (simulation, depth, direction): blockIdx.x is the
     // simulation, blockIdx.y < kLevels the forward scan of depth blockIdx.y and
     // the rest the backward scans.  The scan walks the simulation's tiles 64 at
     // a time; the loads of up to kScanBatch chunks are issued before their scans.
     constexpr uint32_t kScanBatch = 8;
     const SimDesc& S = sim_at(sims, blockIdx.x);
-    const uint32_t T = (S.n + kTopoTile - 1) / kTopoTile;
+    const uint32_t T = (S.n + tile - 1) / tile;  // the set's topology tiles of this simulation
     const uint32_t t0 = S.first_topo_tile;
     const uint32_t lane = lane_id();
     const uint32_t chunks = (T + 63) / 64;
@@ -1147,8 +1152,10 @@ __global__ __launch_bounds__(64) void k_topo_scan(const SimDesc* __restrict__ si
     }
 }
 
-// Boundary queries of one topology tile (see k_topo_pre).
+// Boundary queries of one topology tile of kTT positions (see k_topo_pre).
+template <uint32_t kTT>
 struct TileBounds {
+    static constexpr uint32_t kTopoWaves = kTT / 64;  // 64-position chunks of the tile
     unsigned long long mask[kTopoWaves][kLevels];  // bit l: position wbase + l has dmin <= d
     uint32_t before[kTopoWaves][kLevels];          // one past the last boundary in earlier waves (0: none)
     uint32_t after[kTopoWaves][kLevels];           // first boundary in later waves (kNone: none)
@@ -1211,20 +1218,20 @@ struct TileBounds {
     }
 };
 
+template <uint32_t kTT>
 __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims,
                                                 const TileRef* __restrict__ ttiles,
                                                 uint32_t* __restrict__ ctl,
                                                 unsigned long long* __restrict__ tflags,
                                                 const uint32_t* __restrict__ psum, uint32_t block,
                                                 uint32_t grid) {
-    // Each thread handles positions base + j * kTopoThreads + threadIdx.x, j < kTopoPer,
-    // so a wavefront covers 64 consecutive positions in every slice j and the
-    // per-block latency chain (ticket, tile, look-back) is paid once per 1024
-    // positions by 256 threads.  Per-position state lives in LDS between phases.
+    // One position per thread (the loops over j are the slices of a strided
+    // layout, one slice here).  Per-position state lives in LDS between phases.
+    constexpr uint32_t kTopoTile = kTT, kTopoThreads = kTT, kTopoPer = 1;
     __shared__ uint32_t s_scan[kTopoThreads / 64];
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
-    __shared__ TileBounds tb;
+    __shared__ TileBounds<kTT> tb;
     __shared__ uint32_t s_key[kTopoTile];   // keys of the tile
     __shared__ uint32_t s_pl[kTopoTile];    // start of the parent group of a column head
     __shared__ uint32_t s_mine[kTopoTile];  // exclusive (columns << 16 | cells) prefix in the tile
@@ -1416,12 +1423,13 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     }
 }
 
-__global__ __launch_bounds__(kTopoThreads) void k_topo_count(const SimDesc* __restrict__ sims,
-                                                             const TileRef* __restrict__ ttiles,
-                                                             uint32_t* __restrict__ ctl,
-                                                             unsigned long long* __restrict__ tflags,
-                                                             const uint32_t* __restrict__ psum) {
-    topo_count_body(sims, ttiles, ctl, tflags, psum, blockIdx.x, gridDim.x);
+template <uint32_t kTT>
+__global__ __launch_bounds__(kTT) void k_topo_count(const SimDesc* __restrict__ sims,
+                                                    const TileRef* __restrict__ ttiles,
+                                                    uint32_t* __restrict__ ctl,
+                                                    unsigned long long* __restrict__ tflags,
+                                                    const uint32_t* __restrict__ psum) {
+    topo_count_body<kTT>(sims, ttiles, ctl, tflags, psum, blockIdx.x, gridDim.x);
 }
 
 // Internal-cell aggregate from its children in quadrant order
@@ -1488,19 +1496,20 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
 // bottom-up aggregates of every internal cell whose range lies inside this tile.
 // Internal cells that cross a tile boundary go to the simulation's span list.
 // The tile's cells are a contiguous preorder range [cb0, cb1); when it fits
-// kLinkCells, children links, node words, range ends and f64 aggregates are kept
+// kLinkCells (1.5 cells per position), children links, node words, range ends and f64 aggregates are kept
 // in LDS, so each depth round of the bottom-up pass is an LDS round trip, and
 // only depths present in the tile are visited.  Every internal cell's records
 // (cell, hot node, child links, aggregate) are written whole, once, in its
 // round: no partial updates of lines written by another kernel.  Larger tiles
 // use the same rounds through global memory.
 // ----------------------------------------------------------------------------
-constexpr uint32_t kLinkCells = 1536;
 constexpr uint32_t kLinkBatch = 4;  // cells of a column whose loads are issued together
 
+template <uint32_t kTT>
 __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
                                                const TileRef* __restrict__ ttiles, uint32_t block,
                                                uint32_t grid) {
+    constexpr uint32_t kTopoTile = kTT, kLinkCells = kTT + kTT / 2;
     __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];
     __shared__ uint32_t s_ch[kLinkCells][4];
     __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];
@@ -1677,9 +1686,10 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
 }
 
-__global__ __launch_bounds__(kTopoTile) void k_topo_link(const SimDesc* __restrict__ sims,
-                                                         const TileRef* __restrict__ ttiles) {
-    topo_link_body(sims, ttiles, blockIdx.x, gridDim.x);
+template <uint32_t kTT>
+__global__ __launch_bounds__(kTT) void k_topo_link(const SimDesc* __restrict__ sims,
+                                                   const TileRef* __restrict__ ttiles) {
+    topo_link_body<kTT>(sims, ttiles, blockIdx.x, gridDim.x);
 }
 
 // The whole topology of a set with a single topology tile (one simulation of
@@ -1694,15 +1704,15 @@ __device__ __forceinline__ void topo_single_body(const SimDesc* __restrict__ sim
                                                  uint32_t* __restrict__ tsum, uint32_t* __restrict__ psum,
                                                  uint32_t* __restrict__ errs) {
     const SimDesc& S = sim_at(sims, ttiles[0].sim);
-    topo_pre_body(sims, ttiles, tsum, 0u, 1u);
+    topo_pre_body<kTopoTile>(sims, ttiles, tsum, 0u, 1u);
     if (threadIdx.x < kLevels) {
         psum[threadIdx.x] = 0u;
         psum[kLevels + threadIdx.x] = S.n;
     }
     __syncthreads();
-    topo_count_body(sims, ttiles, ctl, tflags, psum, 0u, 1u);
+    topo_count_body<kTopoTile>(sims, ttiles, ctl, tflags, psum, 0u, 1u);
     __syncthreads();
-    topo_link_body(sims, ttiles, 0u, 1u);
+    topo_link_body<kTopoTile>(sims, ttiles, 0u, 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         if (S.meta->nspan != 0u) atomicOr(&S.meta->err, kErrSpanCapacity);  // cannot happen

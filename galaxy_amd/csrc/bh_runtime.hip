@@ -82,7 +82,8 @@ static_assert(sizeof(NodeHot) == 16 && sizeof(NodeCold) == 16, "node records are
 static_assert(cell_capacity(GBH_MAX_LARGE_TREE_BODIES) + GBH_MAX_LARGE_TREE_BODIES < kMaxNodes,
               "the node stream of the largest tree fits the hot word");
 
-uint32_t span_capacity(uint32_t n) { return 16u * div_up(n, kTopoTile) + 16u; }
+// (for either topology tile size: at most 16 spanning cells per tile edge)
+uint32_t span_capacity(uint32_t n) { return 16u * div_up(n, kTopoTileMulti) + 16u; }
 
 // Layout of one simulation's device arena.  With tree == false only the
 // resident state is placed.
@@ -191,6 +192,7 @@ struct SimSet {
     uint32_t trav_waves_per_cu = 32;  // persistent traversal wavefronts per CU
     uint32_t traverse_blocks = 0;
     uint32_t lane_targets = 1;        // traversal targets per lane (build_tables)
+    uint32_t topo_tile = kTopoTile;   // topology tile positions (build_tables)
 
     ~SimSet() { release(); }
 
@@ -229,6 +231,7 @@ struct SimSet {
         ttopo.clear();
         twave.clear();
         const uint32_t st = sort_tile();
+        topo_tile = desc.size() == 1 ? kTopoTile : kTopoTileMulti;
         for (uint32_t s = 0; s < desc.size(); ++s) {
             SimDesc& d = desc[s];
             d.first_tile = (uint32_t)t256.size();
@@ -236,7 +239,7 @@ struct SimSet {
             d.first_sort_tile = (uint32_t)tsort.size();
             for (uint32_t b = 0; b < d.n; b += st) tsort.push_back({s, b});
             d.first_topo_tile = (uint32_t)ttopo.size();
-            for (uint32_t b = 0; b < d.n; b += kTopoTile) ttopo.push_back({s, b});
+            for (uint32_t b = 0; b < d.n; b += topo_tile) ttopo.push_back({s, b});
         }
         // Traversal work items.  Mode 0 (many simulations): one contiguous range
         // per XCD; simulations are dealt to the XCDs round-robin in size order
@@ -374,17 +377,20 @@ struct SimSet {
             HIP_OK(hipGetLastError());
             return;
         }
-        hipLaunchKernelGGL(k_topo_pre, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
-                           d_ttopo, d_tsum);
+        if (topo_tile == kTopoTileMulti) topo_tiled<kTopoTileMulti>(s);
+        else topo_tiled<kTopoTile>(s);
+    }
+    template <uint32_t kTT>
+    void topo_tiled(hipStream_t s) {
+        const dim3 grid((uint32_t)ttopo.size());
+        hipLaunchKernelGGL(k_topo_pre<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size(), 2 * kLevels), dim3(64), 0, s, d_desc, d_tsum,
-                           d_psum);
+                           d_psum, kTT);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_count, dim3((uint32_t)ttopo.size()), dim3(kTopoThreads), 0, s, d_desc,
-                           d_ttopo, d_ctl, d_tflags, d_psum);
+        hipLaunchKernelGGL(k_topo_count<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo, d_ctl, d_tflags, d_psum);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_link, dim3((uint32_t)ttopo.size()), dim3(kTopoTile), 0, s, d_desc,
-                           d_ttopo);
+        hipLaunchKernelGGL(k_topo_link<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo);
         HIP_OK(hipGetLastError());
     }
     void agg(hipStream_t s) {
