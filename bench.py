@@ -148,7 +148,7 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
     cfg = replace(w.config, bucket=choice.bucket)
     b = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=choice.sort_tile,
               traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu,
-              streams=choice.streams)
+              streams=choice.streams, topo_tile=choice.topo_tile)
     total = w.bodies
     b.step(max(warmup, 1))
     t0 = time.perf_counter()
@@ -176,7 +176,7 @@ def batched_run(gpu, steps, warmup, count=256, tune=True, tune_cache=None, choic
                 one = autotune.Choice(**min(ones, key=lambda x: x[0])[1])
         b1 = Batch(gpu, w.states, cfg, w.dt, use_graph=True, sort_tile=one.sort_tile,
                    traverse_block=one.traverse_block, traverse_waves_per_cu=one.traverse_waves_per_cu,
-                   streams=1)
+                   streams=1, topo_tile=one.topo_tile)
         b1.step(max(warmup, 1))
         t1 = time.perf_counter()
         b1.step(steps)
@@ -218,7 +218,8 @@ def serving_run(gpu, requests=20000, choice=None, threads=16):
     t_gen = time.perf_counter() - t_gen
     log(f"  generated {len(reqs)} request states ({sum(len(s) for s in states)} bodies) in {t_gen:.1f} s")
     body_steps = sum(n * k for _, _, n, _, k in reqs)
-    shape = {k: choice[k] for k in ("sort_tile", "traverse_block", "traverse_waves_per_cu", "streams")
+    shape = {k: choice[k] for k in ("sort_tile", "traverse_block", "traverse_waves_per_cu", "streams",
+                                                 "topo_tile")
              if k in choice} if choice else {}
     srv = Server(gpu, hbm_budget_bytes=200 * 10**9, steps_per_round=10, **shape)
     srv.start()

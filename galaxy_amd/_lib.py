@@ -46,7 +46,8 @@ class SimParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sort_tile", C.c_uint32), ("traverse_block", C.c_uint32),
-                ("traverse_waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("streams", C.c_uint32)]
+                ("traverse_waves_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("streams", C.c_uint32),
+                ("topo_tile", C.c_uint32)]
 
 
 class ServerOptions(C.Structure):

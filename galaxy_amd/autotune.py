@@ -11,6 +11,7 @@ The search space:
   traverse_waves_per_cu  persistent traversal wavefronts per CU: 32, 24, 20, 16, 12
   streams                groups of simulations stepped on streams of their own, so
                          their kernels overlap: 1, 2
+  topo_tile              sorted positions per topology workgroup: 512, 1024
 
 The launch-shape choices leave every result bit-identical; the leaf size defines
 the tree, and the CPU restatement reproduces the tree and forces for any leaf size.
@@ -41,12 +42,14 @@ SPACE = {
     "traverse_block": (256, 128, 64),
     "traverse_waves_per_cu": (32, 24, 20, 16, 12),
     "streams": (1, 2),
+    "topo_tile": (512, 1024),
 }
 
 # Coordinates of the descent.  Resident traversal waves and the number of
 # groups are searched together: two groups overlap well only with fewer
 # traversal waves per CU, so neither improves alone from the one-group optimum.
-COORDINATES = (("bucket",), ("sort_tile",), ("traverse_block",), ("traverse_waves_per_cu", "streams"))
+COORDINATES = (("bucket",), ("sort_tile",), ("topo_tile",), ("traverse_block",),
+               ("traverse_waves_per_cu", "streams"))
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "galaxy_amd", "tune.json")
 
@@ -58,6 +61,7 @@ class Choice:
     traverse_block: int = 256
     traverse_waves_per_cu: int = 32
     streams: int = 1
+    topo_tile: int = 512
 
 
 def signature(sizes, config: Config, dt: float, device: str) -> str:
@@ -76,7 +80,7 @@ def measure(gpu: NbodyGpu, states, config: Config, dt: float, choice: Choice, st
     cfg = replace(config, bucket=choice.bucket)
     b = Batch(gpu, states, cfg, dt, use_graph=True, sort_tile=choice.sort_tile,
               traverse_block=choice.traverse_block, traverse_waves_per_cu=choice.traverse_waves_per_cu,
-              streams=choice.streams)
+              streams=choice.streams, topo_tile=choice.topo_tile)
     try:
         b.step(warmup)
         t0 = time.perf_counter()

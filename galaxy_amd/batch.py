@@ -25,7 +25,7 @@ def _p(a):
 
 class Batch:
     def __init__(self, gpu: NbodyGpu, states, configs, dts, use_graph=True, sort_tile=2048,
-                 traverse_block=256, traverse_waves_per_cu=32, streams=1):
+                 traverse_block=256, traverse_waves_per_cu=32, streams=1, topo_tile=0):
         states = [as_state(s) for s in states]
         if not isinstance(configs, (list, tuple)):
             configs = [configs] * len(states)
@@ -37,7 +37,8 @@ class Batch:
         for i, (c, dt) in enumerate(zip(configs, dts)):
             params[i].config = c.c()
             params[i].dt = float(dt)
-        tuning = L.Tuning(sort_tile, traverse_block, traverse_waves_per_cu, 1 if use_graph else 0, streams)
+        tuning = L.Tuning(sort_tile, traverse_block, traverse_waves_per_cu, 1 if use_graph else 0, streams,
+                          topo_tile)
         h = C.c_void_p()
         _check(L.lib().gbh_batch_create(gpu._h, len(states), _p(self.counts), _p(allst), params,
                                         C.byref(tuning), C.byref(h)))

@@ -998,7 +998,6 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_single(const SimDesc* __r
 // group's traversal), sets of one simulation kTopoTile.
 constexpr uint32_t kTopoTileMulti = 512;
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
-constexpr uint32_t kLeafBatch = 4;           // leaf bodies whose loads are issued together
 constexpr uint32_t kNone = 0xffffffffu;
 
 __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint32_t p, uint32_t kp) {
@@ -1225,17 +1224,20 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
                                                 unsigned long long* __restrict__ tflags,
                                                 const uint32_t* __restrict__ psum, uint32_t block,
                                                 uint32_t grid) {
-    // One position per thread (the loops over j are the slices of a strided
-    // layout, one slice here).  Per-position state lives in LDS between phases.
-    constexpr uint32_t kTopoTile = kTT, kTopoThreads = kTT, kTopoPer = 1;
-    __shared__ uint32_t s_scan[kTopoThreads / 64];
+    // One position per thread for the columns, their scan and the body nodes;
+    // the cells are then written cell-parallel: the tile's leaves (one per
+    // column, listed in position order) and its internal cells (listed in
+    // preorder, in passes of kTT) each get a thread of their own, so a column
+    // of many cells does not hold its wavefront for every other lane.
+    __shared__ uint32_t s_scan[kTT / 64];
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
     __shared__ TileBounds<kTT> tb;
-    __shared__ uint32_t s_key[kTopoTile];   // keys of the tile
-    __shared__ uint32_t s_pl[kTopoTile];    // start of the parent group of a column head
-    __shared__ uint32_t s_mine[kTopoTile];  // exclusive (columns << 16 | cells) prefix in the tile
-    __shared__ uint8_t s_dmin[kTopoTile], s_count[kTopoTile];
+    __shared__ uint32_t s_key[kTT];   // keys of the tile
+    __shared__ uint32_t s_mine[kTT];  // exclusive (columns << 16 | cells) prefix in the tile
+    __shared__ float4 s_body[kTT];    // (x, y, m) at the tile's positions
+    __shared__ uint16_t s_list[kTT];  // column heads, then internal cells (tile offsets of their columns)
+    __shared__ uint8_t s_dmin[kTT], s_count[kTT];
     __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
     // simulation from the block index, tile from a ticket of that simulation
     // (as in k_sort_pass)
@@ -1253,62 +1255,43 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
     const uint32_t* __restrict__ K = S.key0;
     const uint32_t base = tr.base;
-    const uint32_t tile_end = min(base + kTopoTile, n);
-#pragma unroll
-    for (uint32_t j = 0; j < kTopoPer; ++j) {
-        const uint32_t o = j * kTopoThreads + threadIdx.x;
-        const uint32_t p = base + o;
-        s_key[o] = p < n ? K[p] : 0u;
-        s_dmin[o] = p < n ? S.cdmin[p] : (uint8_t)17u;
-    }
+    const uint32_t tile_end = min(base + kTT, n);
+    const uint32_t o = threadIdx.x;
+    const uint32_t p = base + o;
+    const bool valid = p < n;
+    s_key[o] = valid ? K[p] : 0u;
+    s_dmin[o] = valid ? S.cdmin[p] : (uint8_t)17u;
+    const float4 body = S.sbody[min(p, n - 1u)];
+    s_body[o] = body;
     tb.load_summaries(psum + (size_t)t * kTopoSum);
     __syncthreads();
     tb.build(base, n, s_dmin);
     auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
-    // columns: existence from the parent group's size, leaf depth from the key at p + bucket
-    // The first kLeafBatch bodies of each column's leaf are loaded here, so the
-    // look-back below hides their latency (loaded later, behind the column's
-    // stores, they would wait for those stores: one counter orders both).
-    float4 lead[kTopoPer][kLeafBatch];
-#pragma unroll
-    for (uint32_t j = 0; j < kTopoPer; ++j) {
-        const uint32_t o = j * kTopoThreads + threadIdx.x;
-        const uint32_t p = base + o;
-#pragma unroll
-        for (uint32_t k = 0; k < kLeafBatch; ++k) lead[j][k] = S.sbody[min(p + k, n - 1u)];
-        uint32_t count = 0, pl = 0;
-        const uint32_t dmin = s_dmin[o];
-        if (p < n && dmin <= maxd) {
-            const uint32_t kp = s_key[o];
-            bool exists = true;
-            if (dmin >= 1) {
-                const uint32_t dd = dmin - 1;
-                pl = tb.prev(base, p, dd);
-                const uint32_t q = pl + bucket;
-                exists = (q < n) && same_prefix(key_at(q), kp, dd);
-            }
-            if (exists) {
-                const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
-                int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
-                if (dl < (int)dmin) dl = (int)dmin;
-                count = (uint32_t)dl - dmin + 1;
-            }
+    // column: existence from the parent group's size, leaf depth from the key at p + bucket
+    uint32_t count = 0, pl = 0;
+    const uint32_t dmin = s_dmin[o];
+    if (valid && dmin <= maxd) {
+        const uint32_t kp = s_key[o];
+        bool exists = true;
+        if (dmin >= 1) {
+            const uint32_t dd = dmin - 1;
+            pl = tb.prev(base, p, dd);
+            const uint32_t q = pl + bucket;
+            exists = (q < n) && same_prefix(key_at(q), kp, dd);
         }
-        s_count[o] = (uint8_t)count;
-        s_pl[o] = pl;
+        if (exists) {
+            const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
+            int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
+            if (dl < (int)dmin) dl = (int)dmin;
+            count = (uint32_t)dl - dmin + 1;
+        }
     }
-    // cells (low 16 bits) and columns (high 16 bits) in position order, slice by
-    // slice: a tile holds at most 17 * 1024 cells
-    uint32_t total = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kTopoPer; ++j) {
-        const uint32_t o = j * kTopoThreads + threadIdx.x;
-        const uint32_t count = s_count[o];
-        uint32_t part;
-        const uint32_t mine = block_excl_sum<kTopoThreads>(count | ((count > 0 ? 1u : 0u) << 16), s_scan, part);
-        s_mine[o] = total + mine;
-        total += part;
-    }
+    s_count[o] = (uint8_t)count;
+    // cells (low 16 bits) and columns (high 16 bits) in position order: a tile
+    // holds at most 17 kTT cells
+    uint32_t total;
+    const uint32_t mine = block_excl_sum<kTT>(count | ((count > 0 ? 1u : 0u) << 16), s_scan, total);
+    s_mine[o] = mine;
     if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
         unsigned long long* f = tflags + t;
         const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
@@ -1327,7 +1310,7 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
         }
         s_excl = (uint32_t)(excl & 0x7fffffffull);
         s_excl_cols = (uint32_t)(excl >> 31);
-        if (threadIdx.x == 0 && base + kTopoTile >= n) {
+        if (threadIdx.x == 0 && base + kTT >= n) {
             const uint32_t ncells = s_excl + (total & 0xffffu);
             const uint32_t ncols = s_excl_cols + (total >> 16);
             S.meta->ncells = ncells;
@@ -1335,89 +1318,80 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
         }
     }
     __syncthreads();
-    for (uint32_t j = 0; j < kTopoPer; ++j) {
-        const uint32_t o = j * kTopoThreads + threadIdx.x;
-        const uint32_t p = base + o;
-        if (p >= n) break;
-        const uint32_t mine = s_mine[o];
-        const uint32_t count = s_count[o];
-        const uint32_t dmin = s_dmin[o];
-        const uint32_t kp = s_key[o];
-        const uint32_t cbase = s_excl + (mine & 0xffffu);
-        // node index of the column's first cell: internal cells and bodies before p
-        const uint32_t nb = cbase - (s_excl_cols + (mine >> 16)) + p;
+    const uint32_t excl = s_excl, cell_cap = S.cell_cap;
+    // Per position: column bookkeeping and the body's node.  The node stream is
+    // the preorder of internal cells with each leaf replaced by its bodies, so
+    // the body at p follows every internal cell of the columns at or before p:
+    // node nb + count - 1 for a column head, nb (= that of its leaf's head plus
+    // its offset in the leaf) for a position inside a leaf.
+    if (valid) {
+        const uint32_t cbase = excl + (mine & 0xffffu);
+        const uint32_t nb = cbase - (s_excl_cols + (mine >> 16)) + p;  // internal cells and bodies before p
         S.cbase[p] = cbase;
         S.nbase[p] = nb;
         S.ccount[p] = (uint8_t)count;
-        S.cparent[p] = s_pl[o];
-        if (count == 0) continue;
-        if (cbase + count > S.cell_cap) {
-            atomicOr(&S.meta->err, kErrCellCapacity);
-            continue;
+        S.cparent[p] = pl;
+        const uint32_t bn = nb + (count > 0u ? count - 1u : 0u);
+        if (bn < S.node_cap) {
+            S.nodes[bn] = NodeHot{body.x, body.y, body.z, node_word(1u, 0u, S.wlayout)};
+            S.bnode[p] = bn;
         }
-        const uint32_t dleaf = dmin + count - 1u;
-        // (cell geometry is not stored: evidence readbacks derive it from the
-        // cell's start key and depth, as child_bounds does, flat.rs:96-103)
-        for (uint32_t d = dmin; d <= dleaf; ++d) {
-            const uint32_t idx = cbase + (d - dmin);
-            const uint32_t e = tb.next(base, p, d);
-            const bool leaf = (d == dleaf);
-            CellT c;
-            c.start = p;
-            c.end = e;
-            c.skip = 0u;
-            c.info = d | (leaf ? kInfoLeaf : 0u);
-            c.T = s_T[d];
-            uint4 ch = make_uint4(kNoChild, kNoChild, kNoChild, kNoChild);
-            if (leaf) {
-                double mass = 0.0, wx = 0.0, wy = 0.0;
-                const uint32_t nfirst = nb + (count - 1u);  // the leaf's bodies in the node stream
-                // kLeafBatch bodies at a time: loads first, then the stores (the
-                // first batch was loaded before the look-back)
-                for (uint32_t q0 = p; q0 < e; q0 += kLeafBatch) {
-                    float4 bv[kLeafBatch];
-#pragma unroll
-                    for (uint32_t k = 0; k < kLeafBatch; ++k)
-                        bv[k] = q0 == p ? lead[j][k]
-                                        : (q0 + k < e ? S.sbody[q0 + k] : make_float4(0.f, 0.f, 0.f, 0.f));
-#pragma unroll
-                    for (uint32_t k = 0; k < kLeafBatch; ++k) {
-                        const uint32_t q = q0 + k;
-                        if (q >= e) break;
-                        const float4 bq = bv[k];
-                        const double bm = bq.z;
-                        mass = mass + bm;
-                        wx = wx + bm * (double)bq.x;
-                        wy = wy + bm * (double)bq.y;
-                        const uint32_t ni = nfirst + (q - p);
-                        S.nodes[ni] = NodeHot{bq.x, bq.y, bq.z, node_word(1u, 0u, S.wlayout)};
-                        S.bnode[q] = ni;
-                    }
-                }
-                const double comx = wx / mass, comy = wy / mass;
-                S.cagg[idx] = Agg{mass, comx, comy, 0.0};
-                c.comx = (float)comx;
-                c.comy = (float)comy;
-                c.mass = (float)mass;
-            } else {
-                const uint32_t q0 = (kp >> (30u - 2u * d)) & 3u;  // first child: same column, depth d+1
-                ch.x = q0 == 0u ? idx + 1u : kNoChild;
-                ch.y = q0 == 1u ? idx + 1u : kNoChild;
-                ch.z = q0 == 2u ? idx + 1u : kNoChild;
-                ch.w = q0 == 3u ? idx + 1u : kNoChild;
-                c.comx = 0.f;
-                c.comy = 0.f;
-                c.mass = 0.f;
+        if (count > 0u) s_list[mine >> 16] = (uint16_t)o;
+    }
+    __syncthreads();
+    // Leaves, one thread each: the bodies [p, e) summed in f64 in order
+    // (fill_aggregates leaf branch, flat.rs:181-195), from LDS inside the tile.
+    // (cell geometry is not stored: evidence readbacks derive it from the
+    // cell's start key and depth, as child_bounds does, flat.rs:96-103)
+    const uint32_t ncols = total >> 16, nint = (total & 0xffffu) - ncols;
+    if (o < ncols) {
+        const uint32_t oc = s_list[o];
+        const uint32_t pc = base + oc;
+        const uint32_t cnt = s_count[oc], dm = s_dmin[oc];
+        const uint32_t dleaf = dm + cnt - 1u;
+        const uint32_t idx = excl + (s_mine[oc] & 0xffffu) + cnt - 1u;
+        if (idx < cell_cap) {
+            const uint32_t e = tb.next(base, pc, dleaf);
+            double mass = 0.0, wx = 0.0, wy = 0.0;
+            for (uint32_t q = pc; q < e; ++q) {
+                const float4 bq = q < tile_end ? s_body[q - base] : S.sbody[q];
+                const double bm = bq.z;
+                mass = mass + bm;
+                wx = wx + bm * (double)bq.x;
+                wy = wy + bm * (double)bq.y;
             }
-            // internal cells' records are written whole by k_topo_link once their
-            // aggregates are known; here only their range end and first child.
-            // A leaf's range end is in its record and it has no children
-            // (readbacks supply the empty links).
-            if (leaf) {
-                S.cells[idx] = c;
+            const double comx = wx / mass, comy = wy / mass;
+            S.cagg[idx] = Agg{mass, comx, comy, 0.0};
+            S.cells[idx] = CellT{(float)comx, (float)comy, (float)mass, s_T[dleaf], pc, e, 0u, dleaf | kInfoLeaf};
+        } else {
+            atomicOr(&S.meta->err, kErrCellCapacity);
+        }
+    }
+    // Internal cells, one thread each: range end and first child (the same
+    // column, depth d + 1).  Their records are written whole by k_topo_link
+    // once their aggregates are known.  Column o's internal cells are the
+    // tile's internal cells iex .. iex + count - 2 in preorder.
+    for (uint32_t c0 = 0; c0 < nint; c0 += kTT) {
+        __syncthreads();  // the previous list is consumed
+        if (count > 1u) {
+            const uint32_t iex = (mine & 0xffffu) - (mine >> 16);
+            const uint32_t lo = max(iex, c0), hi = min(iex + count - 1u, c0 + kTT);
+            for (uint32_t i = lo; i < hi; ++i) s_list[i - c0] = (uint16_t)o;
+        }
+        __syncthreads();
+        if (o < min(kTT, nint - c0)) {
+            const uint32_t oc = s_list[o];
+            const uint32_t mc = s_mine[oc];
+            const uint32_t k = c0 + o - ((mc & 0xffffu) - (mc >> 16));
+            const uint32_t d = s_dmin[oc] + k;
+            const uint32_t idx = excl + (mc & 0xffffu) + k;
+            if (idx < cell_cap) {
+                const uint32_t q0 = (s_key[oc] >> (30u - 2u * d)) & 3u;  // first child's quadrant
+                S.cend[idx] = tb.next(base, base + oc, d);
+                S.cchild[idx] = make_uint4(q0 == 0u ? idx + 1u : kNoChild, q0 == 1u ? idx + 1u : kNoChild,
+                                           q0 == 2u ? idx + 1u : kNoChild, q0 == 3u ? idx + 1u : kNoChild);
             } else {
-                S.cend[idx] = e;
-                S.cchild[idx] = ch;
+                atomicOr(&S.meta->err, kErrCellCapacity);
             }
         }
     }
@@ -1495,73 +1469,35 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
 // K5: skip links (next cell after each subtree), the remaining child links, and
 // bottom-up aggregates of every internal cell whose range lies inside this tile.
 // Internal cells that cross a tile boundary go to the simulation's span list.
-// The tile's cells are a contiguous preorder range [cb0, cb1); when it fits
-// kLinkCells (1.5 cells per position), children links, node words, range ends and f64 aggregates are kept
-// in LDS, so each depth round of the bottom-up pass is an LDS round trip, and
-// only depths present in the tile are visited.  Every internal cell's records
-// (cell, hot node, child links, aggregate) are written whole, once, in its
-// round: no partial updates of lines written by another kernel.  Larger tiles
-// use the same rounds through global memory.
+// The tile's cells are a contiguous preorder range [cb0, cb1).  When it fits
+// kLinkCells (1.5 cells per position), the work is cell-parallel with the state
+// in LDS: each internal cell gets a thread for its loads, node word and cold
+// record; the tile's internal cells are bucketed by depth, so each round of the
+// bottom-up pass runs exactly the cells of its depth, one per thread, on
+// children links and f64 aggregates held in LDS; and every internal cell's
+// records (cell, hot node, child links, aggregate) are then written whole, once,
+// one cell per thread: no partial updates of lines written by another kernel.
+// Larger tiles run the same rounds per column through global memory.
 // ----------------------------------------------------------------------------
 constexpr uint32_t kLinkBatch = 4;  // cells of a column whose loads are issued together
+constexpr uint16_t kNoLocal = 0xffffu;  // child link in LDS: absent or outside the tile's cells
 
+// The global-memory form (tiles whose cells exceed the LDS stage), one column
+// per thread: p, its column (count cells from preorder index base, node index
+// nb, head depth dmin, key kp) and the parent loads (pl_base, pl_dmin).
 template <uint32_t kTT>
-__device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
-                                               const TileRef* __restrict__ ttiles, uint32_t block,
-                                               uint32_t grid) {
-    constexpr uint32_t kTopoTile = kTT, kLinkCells = kTT + kTT / 2;
-    __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];
-    __shared__ uint32_t s_ch[kLinkCells][4];
-    __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];
-    __shared__ uint32_t s_cb0, s_cb1, s_dlo, s_dhi;
-    const TileRef tr = ttiles[xcd_tile(block, grid)];
-    const SimDesc& S = sim_at(sims, tr.sim);
-    const uint32_t n = S.n;
-    const uint32_t p = tr.base + threadIdx.x;
-    const bool valid = p < n;
-    const uint32_t ncells = S.meta->ncells;
-    const uint32_t nnodes = S.meta->nnodes;
-    const uint32_t tile_end = tr.base + kTopoTile;
-    const uint32_t last = min(n, tile_end) - 1u;
-    uint32_t count = 0, dmin = 17, base = 0, nb = 0;
-    if (valid) {
-        count = S.ccount[p];
-        dmin = S.cdmin[p];
-        base = S.cbase[p];
-        nb = S.nbase[p];
-    }
+__device__ __forceinline__ void topo_link_columns(const SimDesc& S, uint32_t p, uint32_t tile_end, uint32_t count,
+                                                  uint32_t dmin, uint32_t base, uint32_t nb, uint32_t kp,
+                                                  uint32_t pl_base, uint32_t pl_dmin) {
+    __shared__ uint32_t s_dlo, s_dhi;
+    const uint32_t n = S.n, ncells = S.meta->ncells, nnodes = S.meta->nnodes;
     if (threadIdx.x == 0) {
-        s_cb0 = base;  // cells before the tile
         s_dlo = 16u;
         s_dhi = 0u;
     }
-    if (p == last) s_cb1 = base + count;
-    bool ok = count > 0 && base + count <= S.cell_cap;
-    __syncthreads();
-    const uint32_t cb0 = s_cb0, cb1 = s_cb1;
-    const bool in_lds = cb0 <= cb1 && cb1 - cb0 <= kLinkCells;
-    // a consistent scan puts every column of the tile inside [cb0, cb1)
-    if (ok && in_lds && (base < cb0 || base + count > cb1)) {
-        atomicOr(&S.meta->err, kErrCellCapacity);
-        ok = false;
-    }
-    if (!ok) count = 0;
-    if (in_lds)
-        for (uint32_t i = threadIdx.x; i < cb1 - cb0; i += kTopoTile)
-            s_ch[i][0] = s_ch[i][1] = s_ch[i][2] = s_ch[i][3] = kNoChild;
-    __syncthreads();
+    __syncthreads();  // (before the atomics below)
     uint32_t inside = 0;  // bit k: cell base + k is internal and lies inside the tile
-    const uint32_t kp = ok ? S.key0[p] : 0u;
-    if (ok) {
-        // loads that do not depend on the column's cells, issued before any store
-        uint32_t pl = 0, pl_base = 0, pl_dmin = 0;
-        if (dmin >= 1) {
-            pl = S.cparent[p];
-            pl_base = S.cbase[pl];
-            pl_dmin = S.cdmin[pl];
-        }
-        Agg leaf{0.0, 0.0, 0.0, 0.0};
-        if (in_lds) leaf = S.cagg[base + count - 1u];  // the leaf's aggregate (k_topo_count)
+    if (count > 0u) {
         // The column's cells, kLinkBatch at a time: each batch loads its range
         // ends, then their skip targets and thresholds, then stores.  (The
         // compiler cannot move a load across a store to another array, so a
@@ -1577,56 +1513,38 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
                 ev[j] = k0 + j + 1u < count ? e : 0u;
             }
             uint32_t nbe[kLinkBatch];
-            float tr[kLinkBatch];
+            float trr[kLinkBatch];
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
                 nbe[j] = S.nbase[min(ev[j], n - 1u)];
-                tr[j] = S.meta->T[min(dmin + k0 + j, kLevels - 1u)];
+                trr[j] = S.meta->T[min(dmin + k0 + j, kLevels - 1u)];
             }
             // The values are consumed here, all at once: otherwise the compiler
             // moves each load under the condition that selects it and waits for
             // it there, one cell after the other.
             static_assert(kLinkBatch == 4, "the register list below names four cells");
-            asm volatile("" : "+v"(nbe[0]), "+v"(nbe[1]), "+v"(nbe[2]), "+v"(nbe[3]), "+v"(tr[0]), "+v"(tr[1]),
-                         "+v"(tr[2]), "+v"(tr[3]));
+            asm volatile("" : "+v"(nbe[0]), "+v"(nbe[1]), "+v"(nbe[2]), "+v"(nbe[3]), "+v"(trr[0]), "+v"(trr[1]),
+                         "+v"(trr[2]), "+v"(trr[3]));
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
                 const bool internal = k0 + j + 1u < count;
                 sk[j] = internal ? (ev[j] < n ? nbe[j] : nnodes) : 0u;
-                tv[j] = internal ? tr[j] : 0.0f;
+                tv[j] = internal ? trr[j] : 0.0f;
             }
 #pragma unroll
             for (uint32_t j = 0; j < kLinkBatch; ++j) {
                 const uint32_t k = k0 + j;
                 if (k + 1u >= count) break;
-                const uint32_t idx = base + k;
-                const uint32_t d = dmin + k;
                 const uint32_t e = ev[j];
-                const uint32_t ni = nb + k;  // internal: cold node now, the rest after the rounds
-                S.ncold[ni] = NodeCold{p, e - p, tv[j], d};
+                S.ncold[nb + k] = NodeCold{p, e - p, tv[j], dmin + k};
                 if (e <= tile_end) inside |= 1u << k;
-                if (in_lds) {
-                    s_w[idx - cb0] = node_word(sk[j] - ni, d, S.wlayout);  // internal depths are 0..15
-                    s_e[idx - cb0] = e;
-                    s_ch[idx - cb0][(kp >> (30u - 2u * d)) & 3u] = idx + 1u;
-                }
             }
-        }
-        if (in_lds) {
-            const uint32_t r = base + count - 1u - cb0;
-            s_m[r] = leaf.m;
-            s_x[r] = leaf.x;
-            s_y[r] = leaf.y;
         }
         if (dmin >= 1) {
             const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
-            if (pl_dmin <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
-                if (in_lds && parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = base;
-                else (&S.cchild[parent].x)[q] = base;  // a parent outside the tile's LDS range
-            } else {
-                atomicOr(&S.meta->err, kErrCellCapacity);
-            }
+            if (pl_dmin <= dmin - 1u && parent < ncells && parent < S.cell_cap) (&S.cchild[parent].x)[q] = base;
+            else atomicOr(&S.meta->err, kErrCellCapacity);
         }
     }
     // depth range of the tile's internal cells: reduced per wavefront first (an
@@ -1639,50 +1557,191 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
     __syncthreads();
     const uint32_t dlo = s_dlo, dhi = s_dhi;
-    // Bottom-up rounds.  With the tile's cells in LDS a round is LDS traffic
-    // only: a barrier also waits for the wavefront's outstanding global stores,
-    // so the records are written after the last round, each cell once.
     for (int d = (int)dhi; d >= (int)dlo; --d) {
         const uint32_t k = (uint32_t)d - dmin;
         if (count > 1u && (uint32_t)d >= dmin && k + 1u < count) {
             const uint32_t idx = base + k;
-            if (in_lds) {
-                if (inside & (1u << k)) {
-                    double mass = 0.0, wx = 0.0, wy = 0.0;
+            const bool in = (inside & (1u << k)) != 0u;
+            link_write_cell(S, idx, nb + k, p, (uint32_t)d, in, combine_children_if(S, idx, in), S.cend[idx], 0u,
+                            nullptr);
+        }
+        __syncthreads();  // the next depth reads this depth's global stores
+    }
+}
+
+template <uint32_t kTT>
+__device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
+                                               const TileRef* __restrict__ ttiles, uint32_t block,
+                                               uint32_t grid) {
+    constexpr uint32_t kLinkCells = kTT + kTT / 2;
+    static_assert(kLinkCells < kNoLocal && kTT <= 2048, "LDS cell offsets and bucket entries are 16-bit");
+    // per cell of the tile (offset from cb0)
+    __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];  // f64 aggregates
+    __shared__ uint16_t s_ch[kLinkCells][4];                               // children (offsets)
+    __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];                 // node word, range end
+    __shared__ uint16_t s_blist[kLinkCells];  // internal cells bucketed by depth: (column offset << 5) | k
+    // per position of the tile
+    __shared__ uint32_t s_pcb[kTT], s_pnb[kTT], s_pkey[kTT];
+    __shared__ uint8_t s_pdm[kTT];
+    __shared__ uint32_t s_dcnt[kLevels], s_doff[kLevels + 1];
+    __shared__ float s_T[kLevels];
+    __shared__ uint32_t s_cb0, s_cb1;
+    const TileRef tr = ttiles[xcd_tile(block, grid)];
+    const SimDesc& S = sim_at(sims, tr.sim);
+    const uint32_t n = S.n;
+    const uint32_t o = threadIdx.x;
+    const uint32_t p = tr.base + o;
+    const bool valid = p < n;
+    const uint32_t tile_end = tr.base + kTT;
+    const uint32_t last = min(n, tile_end) - 1u;
+    uint32_t count = 0, dmin = 17, base = 0, nb = 0;
+    if (valid) {
+        count = S.ccount[p];
+        dmin = S.cdmin[p];
+        base = S.cbase[p];
+        nb = S.nbase[p];
+    }
+    if (o == 0) s_cb0 = base;  // cells before the tile
+    if (p == last) s_cb1 = base + count;
+    if (o < kLevels) {
+        s_dcnt[o] = 0u;
+        s_T[o] = S.meta->T[o];
+    }
+    bool ok = count > 0 && base + count <= S.cell_cap;
+    // loads that do not depend on the column's cells, issued before any store
+    uint32_t pl_base = 0, pl_dmin = 0;
+    if (ok && dmin >= 1) {
+        const uint32_t pl = S.cparent[p];
+        pl_base = S.cbase[pl];
+        pl_dmin = S.cdmin[pl];
+    }
+    const uint32_t kp = ok ? S.key0[p] : 0u;
+    __syncthreads();
+    const uint32_t cb0 = s_cb0, cb1 = s_cb1;
+    const uint32_t ntile = cb1 - cb0;
+    if (!(cb0 <= cb1 && ntile <= kLinkCells)) {  // (uniform) the tile's cells exceed the LDS stage
+        topo_link_columns<kTT>(S, p, tile_end, ok ? count : 0u, dmin, base, nb, kp, pl_base, pl_dmin);
+        return;
+    }
+    const uint32_t ncells = S.meta->ncells, nnodes = S.meta->nnodes;
+    // a consistent scan puts every column of the tile inside [cb0, cb1)
+    if (ok && (base < cb0 || base + count > cb1)) {
+        atomicOr(&S.meta->err, kErrCellCapacity);
+        ok = false;
+    }
+    if (!ok) count = 0;
+    s_pcb[o] = base;
+    s_pnb[o] = nb;
+    s_pkey[o] = kp;
+    s_pdm[o] = (uint8_t)dmin;
+    for (uint32_t i = o; i < ntile; i += kTT) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint32_t c = s_ch[idx - cb0][q];
-                        if (c != kNoChild) {
-                            const double cm = s_m[c - cb0];
-                            mass = mass + cm;
-                            wx = wx + cm * s_x[c - cb0];
-                            wy = wy + cm * s_y[c - cb0];
-                        }
-                    }
-                    s_m[idx - cb0] = mass;
-                    s_x[idx - cb0] = wx / mass;
-                    s_y[idx - cb0] = wy / mass;
-                }
+        for (int q = 0; q < 4; ++q) s_ch[i][q] = kNoLocal;
+    }
+    Agg leaf{0.0, 0.0, 0.0, 0.0};
+    if (count > 0u) leaf = S.cagg[base + count - 1u];  // the leaf's aggregate (k_topo_count)
+    // rank of each internal cell among the tile's cells of its depth (held in s_e until the list is built)
+    for (uint32_t k = 0; k + 1u < count; ++k) s_e[base - cb0 + k] = atomicAdd(&s_dcnt[dmin + k], 1u);
+    __syncthreads();
+    if (o < 64) {  // bucket offsets: exclusive prefix of the per-depth counts
+        const uint32_t c = o < kLevels ? s_dcnt[o] : 0u;
+        const uint32_t incl = wave_incl_sum_u32(c);
+        if (o <= kLevels) s_doff[o] = incl - c;
+    }
+    __syncthreads();
+    if (count > 0u) {
+        const uint32_t r = base + count - 1u - cb0;
+        s_m[r] = leaf.m;
+        s_x[r] = leaf.x;
+        s_y[r] = leaf.y;
+        for (uint32_t k = 0; k + 1u < count; ++k)
+            s_blist[s_doff[dmin + k] + s_e[base - cb0 + k]] = (uint16_t)((o << 5) | k);
+        if (dmin >= 1) {
+            const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
+            const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
+            if (pl_dmin <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
+                if (parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = (uint16_t)(base - cb0);
+                else (&S.cchild[parent].x)[q] = base;  // a parent outside the tile's cells
             } else {
-                link_write_cell(S, idx, nb + k, p, (uint32_t)d, (inside & (1u << k)) != 0u,
-                                combine_children_if(S, idx, (inside & (1u << k)) != 0u), S.cend[idx],
-                                0u, nullptr);
+                atomicOr(&S.meta->err, kErrCellCapacity);
             }
         }
-        __syncthreads();
     }
-    if (in_lds && count > 1u) {
-        // the column's tile-crossing cells take consecutive span slots, one atomic per column
-        const uint32_t crossing = ((1u << (count - 1u)) - 1u) & ~inside;
-        uint32_t slot = crossing ? atomicAdd(&S.meta->nspan, (uint32_t)__builtin_popcount(crossing)) : 0u;
-        for (uint32_t k = 0; k + 1u < count; ++k) {
-            const uint32_t idx = base + k;
-            const uint32_t r = idx - cb0;
-            const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
-            const bool in = (inside & (1u << k)) != 0u;
-            link_write_cell(S, idx, nb + k, p, dmin + k, in, a, s_e[r], s_w[r], s_ch[r], in ? kNone : slot);
-            if (!in) ++slot;
+    __syncthreads();
+    const uint32_t nint = s_doff[kLevels];
+    // Internal cells, one per thread: range end, skip target, node word, cold
+    // record and the first child (the same column, depth d + 1).
+    for (uint32_t j = o; j < nint; j += kTT) {
+        const uint32_t ent = s_blist[j];
+        const uint32_t oc = ent >> 5, k = ent & 31u;
+        const uint32_t r = s_pcb[oc] - cb0 + k;
+        const uint32_t idx = cb0 + r, ni = s_pnb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
+        const uint32_t e = S.cend[idx];
+        const uint32_t skip = e < n ? S.nbase[e] : nnodes;
+        S.ncold[ni] = NodeCold{pc, e - pc, s_T[d], d};
+        s_w[r] = node_word(skip - ni, d, S.wlayout);  // internal depths are 0..15
+        s_e[r] = e;
+        s_ch[r][(s_pkey[oc] >> (30u - 2u * d)) & 3u] = (uint16_t)(r + 1u);
+    }
+    __syncthreads();
+    // Bottom-up rounds, deepest depth first, over the cells of each depth that
+    // lie inside the tile (fill_aggregates internal branch, flat.rs:196-213):
+    // LDS traffic only, so the barriers need not wait for global stores.
+    for (int d = 15; d >= 0; --d) {
+        const uint32_t lo = s_doff[d], hi = s_doff[d + 1];
+        if (lo == hi) continue;  // (uniform) no internal cell of this depth
+        for (uint32_t j = lo + o; j < hi; j += kTT) {
+            const uint32_t ent = s_blist[j];
+            const uint32_t r = s_pcb[ent >> 5] - cb0 + (ent & 31u);
+            if (s_e[r] > tile_end) continue;  // spanning: k_agg_top
+            double mass = 0.0, wx = 0.0, wy = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t c = s_ch[r][q];
+                if (c != kNoLocal) {
+                    const double cm = s_m[c];
+                    mass = mass + cm;
+                    wx = wx + cm * s_x[c];
+                    wy = wy + cm * s_y[c];
+                }
+            }
+            s_m[r] = mass;
+            s_x[r] = wx / mass;
+            s_y[r] = wy / mass;
         }
+        lds_barrier();
+    }
+    // Records, one internal cell per thread.  A tile-crossing cell takes a span
+    // slot (one atomic per wavefront) and the links to its children that lie in
+    // the tile (those in later tiles were or will be written by their tiles).
+    for (uint32_t j0 = 0; j0 < nint; j0 += kTT) {  // (uniform trip count)
+        const uint32_t j = j0 + o;
+        const bool have = j < nint;
+        uint32_t r = 0, oc = 0, k = 0;
+        if (have) {
+            const uint32_t ent = s_blist[j];
+            oc = ent >> 5;
+            k = ent & 31u;
+            r = s_pcb[oc] - cb0 + k;
+        }
+        const uint32_t e = have ? s_e[r] : 0u;
+        const bool in = e <= tile_end;
+        const unsigned long long span = __ballot(have && !in);
+        uint32_t slot = kNone;
+        if (span) {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(span);
+            uint32_t first = 0u;
+            if (lane_id() == leader) first = atomicAdd(&S.meta->nspan, (uint32_t)__builtin_popcountll(span));
+            first = __shfl(first, (int)leader);
+            slot = first + (uint32_t)__builtin_popcountll(span & ((1ull << lane_id()) - 1ull));
+        }
+        if (!have) continue;
+        uint32_t ch[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ch[q] = s_ch[r][q] == kNoLocal ? kNoChild : cb0 + s_ch[r][q];
+        const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
+        link_write_cell(S, cb0 + r, s_pnb[oc] + k, tr.base + oc, s_pdm[oc] + k, in, a, e, s_w[r], ch,
+                        in ? kNone : slot);
     }
 }
 

@@ -193,6 +193,7 @@ struct SimSet {
     uint32_t traverse_blocks = 0;
     uint32_t lane_targets = 1;        // traversal targets per lane (build_tables)
     uint32_t topo_tile = kTopoTile;   // topology tile positions (build_tables)
+    uint32_t topo_tile_multi = kTopoTileMulti;  // the tile of a set of several simulations
 
     ~SimSet() { release(); }
 
@@ -231,7 +232,7 @@ struct SimSet {
         ttopo.clear();
         twave.clear();
         const uint32_t st = sort_tile();
-        topo_tile = desc.size() == 1 ? kTopoTile : kTopoTileMulti;
+        topo_tile = desc.size() == 1 ? kTopoTile : topo_tile_multi;
         for (uint32_t s = 0; s < desc.size(); ++s) {
             SimDesc& d = desc[s];
             d.first_tile = (uint32_t)t256.size();

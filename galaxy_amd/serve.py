@@ -43,9 +43,9 @@ class Completion:
 class Server:
     def __init__(self, gpu: NbodyGpu, hbm_budget_bytes: int = 200 * 10**9, steps_per_round: int = 10,
                  max_inflight: int = 0, sort_tile: int = 2048, traverse_block: int = 256,
-                 traverse_waves_per_cu: int = 32, streams: int = 1):
+                 traverse_waves_per_cu: int = 32, streams: int = 1, topo_tile: int = 0):
         opt = L.ServerOptions(int(hbm_budget_bytes), int(steps_per_round), int(max_inflight),
-                              L.Tuning(sort_tile, traverse_block, traverse_waves_per_cu, 0, streams))
+                              L.Tuning(sort_tile, traverse_block, traverse_waves_per_cu, 0, streams, topo_tile))
         h = C.c_void_p()
         _check(L.lib().gbh_server_create(gpu._h, C.byref(opt), C.byref(h)))
         self._h = h

@@ -250,7 +250,7 @@ typedef struct {
 } gbh_sim_params;
 
 /* Launch-shape choices; none of them changes any result.  A zero field keeps
- * its default (2048, 256, 32). */
+ * its default (2048, 256, 32, one group, 512). */
 typedef struct {
     uint32_t sort_tile;      /* keys per sort workgroup: 1024 or 2048 */
     uint32_t traverse_block; /* traversal workgroup size: 64, 128, 256 */
@@ -259,6 +259,9 @@ typedef struct {
     uint32_t streams;        /* 1 or 2 groups of simulations (a server: of in-flight
                                 requests), each stepped on a stream of its own so their
                                 kernels overlap (0 = 1) */
+    uint32_t topo_tile;      /* sorted positions per topology workgroup in a set of
+                                several simulations: 512 or 1024 (a set of one
+                                simulation always uses 1024) */
 } gbh_tuning;
 
 /* counts[nsims]; states = all simulations' records concatenated in order. */

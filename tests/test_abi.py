@@ -72,7 +72,7 @@ def test_ctypes_structs_sizes():
     assert C.sizeof(L.StageTiming) == 24
     assert C.sizeof(L.BuildTiming) == 48
     assert C.sizeof(L.SimParams) == 40
-    assert C.sizeof(L.Tuning) == 20
+    assert C.sizeof(L.Tuning) == 24
 
 
 def test_library_exports_every_header_function():
