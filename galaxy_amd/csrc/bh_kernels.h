@@ -997,6 +997,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_single(const SimDesc* __r
 // kTopoTileMulti positions (workgroups small enough to run beside the other
 // group's traversal), sets of one simulation kTopoTile.
 constexpr uint32_t kTopoTileMulti = 512;
+constexpr uint32_t kTopoPassThreads = 512;  // threads of the tiled boundary and column passes
 constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
 constexpr uint32_t kNone = 0xffffffffu;
 
@@ -1012,11 +1013,15 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 // (x, y, m) of the body at sorted position p: its reads go to random body
 // indices (inside the last radix pass their latency held every tile's
 // workgroup), here they overlap the boundary work.
-template <uint32_t kTT>
+template <uint32_t kTT, uint32_t kThreads = kTT>
 __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
                                               const TileRef* __restrict__ ttiles,
                                               uint32_t* __restrict__ tsum, uint32_t block, uint32_t grid) {
-    constexpr uint32_t kTopoThreads = kTT, kTopoPer = 1;
+    // kTT positions on kThreads threads: position j * kThreads + threadIdx.x of
+    // the tile for j < kTopoPer, so a wavefront covers 64 consecutive positions
+    // in every slice j.
+    constexpr uint32_t kTopoThreads = kThreads, kTopoPer = kTT / kThreads;
+    static_assert(kTT % kThreads == 0 && kThreads % 64 == 0, "whole wavefronts per slice");
     // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
     __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
     const uint32_t t = xcd_tile(block, grid);
@@ -1059,7 +1064,7 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
         if (p < n) S.cdmin[p] = (uint8_t)dmin[j];
     }
-    __syncthreads();
+    lds_barrier();  // (orders LDS only: the cdmin stores need not complete)
     if (threadIdx.x < kLevels) {
         uint32_t first = kNone, last = 0u;
         for (uint32_t k = 0; k < kTopoThreads / 64; ++k) {
@@ -1076,11 +1081,11 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
     }
 }
 
-template <uint32_t kTT>
-__global__ __launch_bounds__(kTT) void k_topo_pre(const SimDesc* __restrict__ sims,
-                                                  const TileRef* __restrict__ ttiles,
-                                                  uint32_t* __restrict__ tsum) {
-    topo_pre_body<kTT>(sims, ttiles, tsum, blockIdx.x, gridDim.x);
+template <uint32_t kTT, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_topo_pre(const SimDesc* __restrict__ sims,
+                                                       const TileRef* __restrict__ ttiles,
+                                                       uint32_t* __restrict__ tsum) {
+    topo_pre_body<kTT, kThreads>(sims, ttiles, tsum, blockIdx.x, gridDim.x);
 }
 
 // Inclusive wavefront scan with max / min (all lanes active).
@@ -1180,7 +1185,7 @@ struct TileBounds {
                 if (lane_id() == 0) mask[c][d] = m;
             }
         }
-        __syncthreads();
+        lds_barrier();
         if (threadIdx.x < kLevels) {
             const uint32_t d = threadIdx.x;
             uint32_t b = 0;
@@ -1196,7 +1201,7 @@ struct TileBounds {
                 if (m) a = base + (k - 1) * 64u + (uint32_t)__builtin_ctzll(m);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // Last boundary at depth d at or before position x (x in the tile).
@@ -1217,19 +1222,24 @@ struct TileBounds {
     }
 };
 
-template <uint32_t kTT>
+template <uint32_t kTT, uint32_t kThreads = kTT>
 __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims,
                                                 const TileRef* __restrict__ ttiles,
                                                 uint32_t* __restrict__ ctl,
                                                 unsigned long long* __restrict__ tflags,
                                                 const uint32_t* __restrict__ psum, uint32_t block,
                                                 uint32_t grid) {
-    // One position per thread for the columns, their scan and the body nodes;
-    // the cells are then written cell-parallel: the tile's leaves (one per
-    // column, listed in position order) and its internal cells (listed in
-    // preorder, in passes of kTT) each get a thread of their own, so a column
-    // of many cells does not hold its wavefront for every other lane.
-    __shared__ uint32_t s_scan[kTT / 64];
+    // kTT positions on kThreads threads, kPer per thread (position j * kThreads
+    // + threadIdx.x of the tile for j < kPer) for the columns, their scan and
+    // the body nodes; the cells are then written cell-parallel: the tile's
+    // leaves (one per column, listed in position order) and its internal cells
+    // (listed in preorder, in passes of kTT) each get a thread of their own, so
+    // a column of many cells does not hold its wavefront for every other lane.
+    // More positions per thread spread the workgroup's fixed latency chain
+    // (ticket, loads, look-back) over more of them.
+    constexpr uint32_t kPer = kTT / kThreads;
+    static_assert(kTT % kThreads == 0 && kThreads % 64 == 0, "whole wavefronts per slice");
+    __shared__ uint32_t s_scan[kThreads / 64];
     __shared__ uint32_t s_ticket;
     __shared__ uint32_t s_excl, s_excl_cols;
     __shared__ TileBounds<kTT> tb;
@@ -1244,7 +1254,7 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     const uint32_t sim = ttiles[block].sim;  // (an XCD-contiguous order measured slower here)
     const SimDesc& S = sim_at(sims, sim);
     if (threadIdx.x == 0) s_ticket = S.first_topo_tile + atomicAdd(&S.meta->ticket[4], 1u);
-    __syncthreads();
+    lds_barrier();
     const uint32_t t = min(s_ticket, grid - 1u);
     const TileRef tr = ttiles[t];
     if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
@@ -1256,42 +1266,55 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     const uint32_t* __restrict__ K = S.key0;
     const uint32_t base = tr.base;
     const uint32_t tile_end = min(base + kTT, n);
-    const uint32_t o = threadIdx.x;
-    const uint32_t p = base + o;
-    const bool valid = p < n;
-    s_key[o] = valid ? K[p] : 0u;
-    s_dmin[o] = valid ? S.cdmin[p] : (uint8_t)17u;
-    const float4 body = S.sbody[min(p, n - 1u)];
-    s_body[o] = body;
+    float4 body[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t o = j * kThreads + threadIdx.x, p = base + o;
+        s_key[o] = p < n ? K[p] : 0u;
+        s_dmin[o] = p < n ? S.cdmin[p] : (uint8_t)17u;
+        body[j] = S.sbody[min(p, n - 1u)];
+        s_body[o] = body[j];
+    }
     tb.load_summaries(psum + (size_t)t * kTopoSum);
-    __syncthreads();
+    lds_barrier();
     tb.build(base, n, s_dmin);
     auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
-    // column: existence from the parent group's size, leaf depth from the key at p + bucket
-    uint32_t count = 0, pl = 0;
-    const uint32_t dmin = s_dmin[o];
-    if (valid && dmin <= maxd) {
-        const uint32_t kp = s_key[o];
-        bool exists = true;
-        if (dmin >= 1) {
-            const uint32_t dd = dmin - 1;
-            pl = tb.prev(base, p, dd);
-            const uint32_t q = pl + bucket;
-            exists = (q < n) && same_prefix(key_at(q), kp, dd);
+    // columns: existence from the parent group's size, leaf depth from the key at p + bucket
+    uint32_t count[kPer], pl[kPer], mine[kPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t o = j * kThreads + threadIdx.x, p = base + o;
+        count[j] = 0;
+        pl[j] = 0;
+        const uint32_t dmin = s_dmin[o];
+        if (p < n && dmin <= maxd) {
+            const uint32_t kp = s_key[o];
+            bool exists = true;
+            if (dmin >= 1) {
+                const uint32_t dd = dmin - 1;
+                pl[j] = tb.prev(base, p, dd);
+                const uint32_t q = pl[j] + bucket;
+                exists = (q < n) && same_prefix(key_at(q), kp, dd);
+            }
+            if (exists) {
+                const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
+                int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
+                if (dl < (int)dmin) dl = (int)dmin;
+                count[j] = (uint32_t)dl - dmin + 1;
+            }
         }
-        if (exists) {
-            const int L = (p + bucket < n) ? (int)lcp2(kp, key_at(p + bucket)) : -1;
-            int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
-            if (dl < (int)dmin) dl = (int)dmin;
-            count = (uint32_t)dl - dmin + 1;
-        }
+        s_count[o] = (uint8_t)count[j];
     }
-    s_count[o] = (uint8_t)count;
-    // cells (low 16 bits) and columns (high 16 bits) in position order: a tile
-    // holds at most 17 kTT cells
-    uint32_t total;
-    const uint32_t mine = block_excl_sum<kTT>(count | ((count > 0 ? 1u : 0u) << 16), s_scan, total);
-    s_mine[o] = mine;
+    // cells (low 16 bits) and columns (high 16 bits) in position order, slice by
+    // slice: a tile holds at most 17 kTT cells
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        uint32_t part;
+        mine[j] = total + block_excl_sum<kThreads>(count[j] | ((count[j] > 0 ? 1u : 0u) << 16), s_scan, part);
+        s_mine[j * kThreads + threadIdx.x] = mine[j];
+        total += part;
+    }
     if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
         unsigned long long* f = tflags + t;
         const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
@@ -1317,35 +1340,39 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
             S.meta->nnodes = ncells - ncols + n;  // internal cells + bodies
         }
     }
-    __syncthreads();
+    // (the barriers from here on order LDS only: no wait for the global stores)
+    lds_barrier();
     const uint32_t excl = s_excl, cell_cap = S.cell_cap;
     // Per position: column bookkeeping and the body's node.  The node stream is
     // the preorder of internal cells with each leaf replaced by its bodies, so
     // the body at p follows every internal cell of the columns at or before p:
     // node nb + count - 1 for a column head, nb (= that of its leaf's head plus
     // its offset in the leaf) for a position inside a leaf.
-    if (valid) {
-        const uint32_t cbase = excl + (mine & 0xffffu);
-        const uint32_t nb = cbase - (s_excl_cols + (mine >> 16)) + p;  // internal cells and bodies before p
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t o = j * kThreads + threadIdx.x, p = base + o;
+        if (p >= n) break;
+        const uint32_t cbase = excl + (mine[j] & 0xffffu);
+        const uint32_t nb = cbase - (s_excl_cols + (mine[j] >> 16)) + p;  // internal cells and bodies before p
         S.cbase[p] = cbase;
         S.nbase[p] = nb;
-        S.ccount[p] = (uint8_t)count;
-        S.cparent[p] = pl;
-        const uint32_t bn = nb + (count > 0u ? count - 1u : 0u);
+        S.ccount[p] = (uint8_t)count[j];
+        S.cparent[p] = pl[j];
+        const uint32_t bn = nb + (count[j] > 0u ? count[j] - 1u : 0u);
         if (bn < S.node_cap) {
-            S.nodes[bn] = NodeHot{body.x, body.y, body.z, node_word(1u, 0u, S.wlayout)};
+            S.nodes[bn] = NodeHot{body[j].x, body[j].y, body[j].z, node_word(1u, 0u, S.wlayout)};
             S.bnode[p] = bn;
         }
-        if (count > 0u) s_list[mine >> 16] = (uint16_t)o;
+        if (count[j] > 0u) s_list[mine[j] >> 16] = (uint16_t)o;
     }
-    __syncthreads();
+    lds_barrier();
     // Leaves, one thread each: the bodies [p, e) summed in f64 in order
     // (fill_aggregates leaf branch, flat.rs:181-195), from LDS inside the tile.
     // (cell geometry is not stored: evidence readbacks derive it from the
     // cell's start key and depth, as child_bounds does, flat.rs:96-103)
     const uint32_t ncols = total >> 16, nint = (total & 0xffffu) - ncols;
-    if (o < ncols) {
-        const uint32_t oc = s_list[o];
+    for (uint32_t jl = threadIdx.x; jl < ncols; jl += kThreads) {
+        const uint32_t oc = s_list[jl];
         const uint32_t pc = base + oc;
         const uint32_t cnt = s_count[oc], dm = s_dmin[oc];
         const uint32_t dleaf = dm + cnt - 1u;
@@ -1369,20 +1396,24 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     }
     // Internal cells, one thread each: range end and first child (the same
     // column, depth d + 1).  Their records are written whole by k_topo_link
-    // once their aggregates are known.  Column o's internal cells are the
-    // tile's internal cells iex .. iex + count - 2 in preorder.
+    // once their aggregates are known.  The internal cells of the column at
+    // offset o are the tile's internal cells iex .. iex + count - 2 in preorder.
     for (uint32_t c0 = 0; c0 < nint; c0 += kTT) {
-        __syncthreads();  // the previous list is consumed
-        if (count > 1u) {
-            const uint32_t iex = (mine & 0xffffu) - (mine >> 16);
-            const uint32_t lo = max(iex, c0), hi = min(iex + count - 1u, c0 + kTT);
-            for (uint32_t i = lo; i < hi; ++i) s_list[i - c0] = (uint16_t)o;
+        lds_barrier();  // the previous list is consumed
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            if (count[j] > 1u) {
+                const uint32_t iex = (mine[j] & 0xffffu) - (mine[j] >> 16);
+                const uint32_t lo = max(iex, c0), hi = min(iex + count[j] - 1u, c0 + kTT);
+                for (uint32_t i = lo; i < hi; ++i) s_list[i - c0] = (uint16_t)(j * kThreads + threadIdx.x);
+            }
         }
-        __syncthreads();
-        if (o < min(kTT, nint - c0)) {
-            const uint32_t oc = s_list[o];
+        lds_barrier();
+        const uint32_t m = min(kTT, nint - c0);
+        for (uint32_t jc = threadIdx.x; jc < m; jc += kThreads) {
+            const uint32_t oc = s_list[jc];
             const uint32_t mc = s_mine[oc];
-            const uint32_t k = c0 + o - ((mc & 0xffffu) - (mc >> 16));
+            const uint32_t k = c0 + jc - ((mc & 0xffffu) - (mc >> 16));
             const uint32_t d = s_dmin[oc] + k;
             const uint32_t idx = excl + (mc & 0xffffu) + k;
             if (idx < cell_cap) {
@@ -1397,13 +1428,13 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     }
 }
 
-template <uint32_t kTT>
-__global__ __launch_bounds__(kTT) void k_topo_count(const SimDesc* __restrict__ sims,
-                                                    const TileRef* __restrict__ ttiles,
-                                                    uint32_t* __restrict__ ctl,
-                                                    unsigned long long* __restrict__ tflags,
-                                                    const uint32_t* __restrict__ psum) {
-    topo_count_body<kTT>(sims, ttiles, ctl, tflags, psum, blockIdx.x, gridDim.x);
+template <uint32_t kTT, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_topo_count(const SimDesc* __restrict__ sims,
+                                                         const TileRef* __restrict__ ttiles,
+                                                         uint32_t* __restrict__ ctl,
+                                                         unsigned long long* __restrict__ tflags,
+                                                         const uint32_t* __restrict__ psum) {
+    topo_count_body<kTT, kThreads>(sims, ttiles, ctl, tflags, psum, blockIdx.x, gridDim.x);
 }
 
 // Internal-cell aggregate from its children in quadrant order
@@ -1579,9 +1610,10 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];  // f64 aggregates
     __shared__ uint16_t s_ch[kLinkCells][4];                               // children (offsets)
     __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];                 // node word, range end
-    __shared__ uint16_t s_blist[kLinkCells];  // internal cells bucketed by depth: (column offset << 5) | k
+    __shared__ uint16_t s_blist[kLinkCells];  // internal cells bucketed by depth: cell offset
+    __shared__ uint16_t s_bcol[kLinkCells];   // ... and (column offset << 5) | k
     // per position of the tile
-    __shared__ uint32_t s_pcb[kTT], s_pnb[kTT], s_pkey[kTT];
+    __shared__ uint32_t s_pnb[kTT], s_pkey[kTT];
     __shared__ uint8_t s_pdm[kTT];
     __shared__ uint32_t s_dcnt[kLevels], s_doff[kLevels + 1];
     __shared__ float s_T[kLevels];
@@ -1616,7 +1648,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         pl_dmin = S.cdmin[pl];
     }
     const uint32_t kp = ok ? S.key0[p] : 0u;
-    __syncthreads();
+    lds_barrier();  // (LDS only: the parent loads stay in flight until the links below)
     const uint32_t cb0 = s_cb0, cb1 = s_cb1;
     const uint32_t ntile = cb1 - cb0;
     if (!(cb0 <= cb1 && ntile <= kLinkCells)) {  // (uniform) the tile's cells exceed the LDS stage
@@ -1630,7 +1662,6 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         ok = false;
     }
     if (!ok) count = 0;
-    s_pcb[o] = base;
     s_pnb[o] = nb;
     s_pkey[o] = kp;
     s_pdm[o] = (uint8_t)dmin;
@@ -1642,20 +1673,24 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     if (count > 0u) leaf = S.cagg[base + count - 1u];  // the leaf's aggregate (k_topo_count)
     // rank of each internal cell among the tile's cells of its depth (held in s_e until the list is built)
     for (uint32_t k = 0; k + 1u < count; ++k) s_e[base - cb0 + k] = atomicAdd(&s_dcnt[dmin + k], 1u);
-    __syncthreads();
+    // (the barriers from here on order LDS only: no wait for the global stores)
+    lds_barrier();
     if (o < 64) {  // bucket offsets: exclusive prefix of the per-depth counts
         const uint32_t c = o < kLevels ? s_dcnt[o] : 0u;
         const uint32_t incl = wave_incl_sum_u32(c);
         if (o <= kLevels) s_doff[o] = incl - c;
     }
-    __syncthreads();
+    lds_barrier();
     if (count > 0u) {
         const uint32_t r = base + count - 1u - cb0;
         s_m[r] = leaf.m;
         s_x[r] = leaf.x;
         s_y[r] = leaf.y;
-        for (uint32_t k = 0; k + 1u < count; ++k)
-            s_blist[s_doff[dmin + k] + s_e[base - cb0 + k]] = (uint16_t)((o << 5) | k);
+        for (uint32_t k = 0; k + 1u < count; ++k) {
+            const uint32_t j = s_doff[dmin + k] + s_e[base - cb0 + k];
+            s_blist[j] = (uint16_t)(base - cb0 + k);
+            s_bcol[j] = (uint16_t)((o << 5) | k);
+        }
         if (dmin >= 1) {
             const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
@@ -1667,14 +1702,14 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t nint = s_doff[kLevels];
     // Internal cells, one per thread: range end, skip target, node word, cold
     // record and the first child (the same column, depth d + 1).
     for (uint32_t j = o; j < nint; j += kTT) {
-        const uint32_t ent = s_blist[j];
+        const uint32_t ent = s_bcol[j];
         const uint32_t oc = ent >> 5, k = ent & 31u;
-        const uint32_t r = s_pcb[oc] - cb0 + k;
+        const uint32_t r = s_blist[j];
         const uint32_t idx = cb0 + r, ni = s_pnb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
         const uint32_t e = S.cend[idx];
         const uint32_t skip = e < n ? S.nbase[e] : nnodes;
@@ -1683,7 +1718,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         s_e[r] = e;
         s_ch[r][(s_pkey[oc] >> (30u - 2u * d)) & 3u] = (uint16_t)(r + 1u);
     }
-    __syncthreads();
+    lds_barrier();
     // Bottom-up rounds, deepest depth first, over the cells of each depth that
     // lie inside the tile (fill_aggregates internal branch, flat.rs:196-213):
     // LDS traffic only, so the barriers need not wait for global stores.
@@ -1691,8 +1726,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         const uint32_t lo = s_doff[d], hi = s_doff[d + 1];
         if (lo == hi) continue;  // (uniform) no internal cell of this depth
         for (uint32_t j = lo + o; j < hi; j += kTT) {
-            const uint32_t ent = s_blist[j];
-            const uint32_t r = s_pcb[ent >> 5] - cb0 + (ent & 31u);
+            const uint32_t r = s_blist[j];
             if (s_e[r] > tile_end) continue;  // spanning: k_agg_top
             double mass = 0.0, wx = 0.0, wy = 0.0;
 #pragma unroll
@@ -1719,10 +1753,10 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         const bool have = j < nint;
         uint32_t r = 0, oc = 0, k = 0;
         if (have) {
-            const uint32_t ent = s_blist[j];
+            const uint32_t ent = s_bcol[j];
             oc = ent >> 5;
             k = ent & 31u;
-            r = s_pcb[oc] - cb0 + k;
+            r = s_blist[j];
         }
         const uint32_t e = have ? s_e[r] : 0u;
         const bool in = e <= tile_end;

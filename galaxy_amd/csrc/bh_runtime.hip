@@ -381,15 +381,20 @@ struct SimSet {
         if (topo_tile == kTopoTileMulti) topo_tiled<kTopoTileMulti>(s);
         else topo_tiled<kTopoTile>(s);
     }
+    // Tiles of kTT positions; the boundary pass and the column pass run them on
+    // kTopoPassThreads threads (two positions per thread in 1,024-position
+    // tiles), the link pass on kTT.
     template <uint32_t kTT>
     void topo_tiled(hipStream_t s) {
+        constexpr uint32_t kPT = kTopoPassThreads;
         const dim3 grid((uint32_t)ttopo.size());
-        hipLaunchKernelGGL(k_topo_pre<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo, d_tsum);
+        hipLaunchKernelGGL((k_topo_pre<kTT, kPT>), grid, dim3(kPT), 0, s, d_desc, d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size(), 2 * kLevels), dim3(64), 0, s, d_desc, d_tsum,
                            d_psum, kTT);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_count<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo, d_ctl, d_tflags, d_psum);
+        hipLaunchKernelGGL((k_topo_count<kTT, kPT>), grid, dim3(kPT), 0, s, d_desc, d_ttopo, d_ctl, d_tflags,
+                           d_psum);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_topo_link<kTT>, grid, dim3(kTT), 0, s, d_desc, d_ttopo);
         HIP_OK(hipGetLastError());
