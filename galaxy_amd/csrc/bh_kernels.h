@@ -855,15 +855,19 @@ __device__ __forceinline__ void sort_pass_body(const SimDesc* __restrict__ sims,
     // serialises on one address.)
     const uint32_t sim = stiles[xcd_tile(block, grid)].sim;
     const SimDesc& S = sim_at(sims, sim);
-    if (threadIdx.x == 0) s_ticket = S.first_sort_tile + atomicAdd(&S.meta->ticket[pass], 1u);
+    if (threadIdx.x == 0) s_ticket = atomicAdd(&S.meta->ticket[pass], 1u);
     for (uint32_t k = threadIdx.x; k < kWaves * kRadix; k += kSortThreads) (&s_wcnt[0][0])[k] = 0u;
     __syncthreads();
-    const uint32_t t = min(s_ticket, nstiles - 1u);
-    const TileRef tr = stiles[t];
-    if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
+    // the simulation's tiles are consecutive and of kTileKeys keys, so the
+    // ticket gives the tile without a load of the tile table
+    constexpr uint32_t kTileKeys = kSortThreads * kKPT;
+    const uint32_t local = s_ticket;
+    if (local >= (S.n + kTileKeys - 1u) / kTileKeys) {  // tickets not reset: no tile of this simulation is left
         if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
         return;
     }
+    const uint32_t t = S.first_sort_tile + local;
+    const TileRef tr{sim, local * kTileKeys};
     const uint32_t n = S.n;
     const uint32_t shift = 8u * pass;
     const uint32_t* __restrict__ kin = (pass & 1u) ? S.key1 : S.key0;
@@ -998,14 +1002,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_single(const SimDesc* __r
 // group's traversal), sets of one simulation kTopoTile.
 constexpr uint32_t kTopoTileMulti = 512;
 constexpr uint32_t kTopoPassThreads = 512;  // threads of the tiled boundary and column passes
-constexpr uint32_t kTopoSum = 2 * kLevels;  // per-tile summary words
+constexpr uint32_t kTopoSum = 2 * kLevels + 2;  // per-tile summary words: boundaries by depth, cells, columns
+constexpr uint32_t kLocalBucket = 32;           // leaf sizes whose columns the boundary pass counts
 constexpr uint32_t kNone = 0xffffffffu;
-
-__device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint32_t p, uint32_t kp) {
-    if (p == 0) return 0u;
-    const uint32_t kq = K[p - 1];
-    return kq == kp ? 17u : lcp2(kq, kp) + 1u;
-}
 
 // Per topology tile: dmin of every position (cdmin) and, per depth d, the first
 // boundary position in the tile and one past the last (tsum[t][d], tsum[t][17+d];
@@ -1013,6 +1012,19 @@ __device__ __forceinline__ uint32_t dmin_of(const uint32_t* __restrict__ K, uint
 // (x, y, m) of the body at sorted position p: its reads go to random body
 // indices (inside the last radix pass their latency held every tile's
 // workgroup), here they overlap the boundary work.
+//
+// With a leaf size of at most kLocalBucket it also counts each position's
+// column of cells (ccount) and the tile's cells and columns (tsum[t][34],
+// tsum[t][35]), so that k_topo_scan can give every tile its place in the
+// preorder and the column pass needs no look-back.  The column at p exists iff
+// its head depth dmin <= maxd and, for dmin >= 1, the depth dmin - 1 group
+// holding p has more than `bucket` positions; that group is [pl, e) with
+// pl < p < e, and it has more than `bucket` positions iff some window
+// [q, q + bucket] containing p lies inside it, i.e. iff
+// lcp(key[q], key[q + bucket]) >= dmin - 1 for some q in
+// [max(0, p - bucket), min(p, n - 1 - bucket)] (sorted keys: the two ends of a
+// window share a prefix iff all of it does).  The leaf depth comes from the key
+// at p + bucket as in the column pass.
 template <uint32_t kTT, uint32_t kThreads = kTT>
 __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
                                               const TileRef* __restrict__ ttiles,
@@ -1021,25 +1033,40 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
     // the tile for j < kTopoPer, so a wavefront covers 64 consecutive positions
     // in every slice j.
     constexpr uint32_t kTopoThreads = kThreads, kTopoPer = kTT / kThreads;
+    constexpr uint32_t kHalo = kLocalBucket;
     static_assert(kTT % kThreads == 0 && kThreads % 64 == 0, "whole wavefronts per slice");
     // per wavefront and depth: first boundary, last boundary + 1 (no LDS atomics)
     __shared__ uint32_t s_first[kTopoThreads / 64][kLevels], s_last[kTopoThreads / 64][kLevels];
+    __shared__ uint32_t s_k[kTT + 2 * kHalo];  // keys at base - kHalo .. base + kTT + kHalo - 1
+    __shared__ uint32_t s_tot[kTopoThreads / 64][2];
     const uint32_t t = xcd_tile(block, grid);
     const TileRef tr = ttiles[t];
     const SimDesc& S = sim_at(sims, tr.sim);
-    const uint32_t n = S.n;
+    const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
+    const bool local = bucket <= kLocalBucket;
     const uint32_t w = threadIdx.x >> 6;
-    uint32_t dmin[kTopoPer];
+    // every load before the first store: the keys around the tile, then the body gather
+    for (uint32_t i = threadIdx.x; i < kTT + 2 * kHalo; i += kTopoThreads) {
+        const uint32_t q = tr.base + i - kHalo;
+        s_k[i] = tr.base + i >= kHalo && q < n ? S.key0[q] : 0u;
+    }
     float2 bq[kTopoPer];
     float bm[kTopoPer];
-    // every load before the first store: keys, then the body gather
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
         const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
         const uint32_t v = S.val0[min(p, n - 1u)];
-        dmin[j] = p < n ? dmin_of(S.key0, p, S.key0[p]) : 17u;
         bq[j] = S.pos[v];
         bm[j] = S.mass[v];
+    }
+    lds_barrier();
+    auto K = [&](uint32_t q) { return s_k[q - tr.base + kHalo]; };  // q in [base - kHalo, base + kTT + kHalo)
+    uint32_t dmin[kTopoPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kTopoPer; ++j) {
+        const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+        const uint32_t kp = K(p);
+        dmin[j] = p >= n ? 17u : p == 0 ? 0u : K(p - 1u) == kp ? 17u : lcp2(K(p - 1u), kp) + 1u;
     }
     for (uint32_t d = 0; d < kLevels; ++d) {
         uint32_t first = kNone, last = 0u;
@@ -1058,6 +1085,43 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
             s_last[w][d] = last;
         }
     }
+    uint32_t cells = 0, cols = 0;
+    if (local) {
+#pragma unroll
+        for (uint32_t j = 0; j < kTopoPer; ++j) {
+            const uint32_t p = tr.base + j * kTopoThreads + threadIdx.x;
+            uint32_t count = 0;
+            if (p < n && dmin[j] <= maxd) {
+                const uint32_t kp = K(p);
+                bool exists = dmin[j] == 0u;
+                if (!exists && n > bucket) {
+                    const uint32_t dd = dmin[j] - 1u;
+                    const uint32_t qhi = min(p, n - 1u - bucket);
+                    for (uint32_t q = p >= bucket ? p - bucket : 0u; q <= qhi; ++q) {
+                        if (lcp2(K(q), K(q + bucket)) >= dd) {
+                            exists = true;
+                            break;
+                        }
+                    }
+                }
+                if (exists) {
+                    const int L = (p + bucket < n) ? (int)lcp2(kp, K(p + bucket)) : -1;
+                    int dl = L + 1 < (int)maxd ? L + 1 : (int)maxd;
+                    if (dl < (int)dmin[j]) dl = (int)dmin[j];
+                    count = (uint32_t)dl - dmin[j] + 1;
+                }
+            }
+            if (p < n) S.ccount[p] = (uint8_t)count;
+            cells += count;
+            cols += count > 0u ? 1u : 0u;
+        }
+        cells = wave_incl_sum_u32(cells);
+        cols = wave_incl_sum_u32(cols);
+        if (lane_id() == 63) {
+            s_tot[w][0] = cells;
+            s_tot[w][1] = cols;
+        }
+    }
     // the per-position store comes after the ballots; the barrier below waits for it
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
@@ -1073,6 +1137,11 @@ __device__ __forceinline__ void topo_pre_body(const SimDesc* __restrict__ sims,
         }
         tsum[(size_t)t * kTopoSum + threadIdx.x] = first;
         tsum[(size_t)t * kTopoSum + kLevels + threadIdx.x] = last;
+    } else if (local && threadIdx.x < kLevels + 2) {
+        const uint32_t c = threadIdx.x - kLevels;
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < kTopoThreads / 64; ++k) v += s_tot[k][c];
+        tsum[(size_t)t * kTopoSum + 2 * kLevels + c] = v;
     }
 #pragma unroll
     for (uint32_t j = 0; j < kTopoPer; ++j) {
@@ -1122,6 +1191,26 @@ __global__ __launch_bounds__(64) void k_topo_scan(const SimDesc* __restrict__ si
     const uint32_t t0 = S.first_topo_tile;
     const uint32_t lane = lane_id();
     const uint32_t chunks = (T + 63) / 64;
+    if (blockIdx.y == 2 * kLevels) {
+        // The cells and columns before each tile (exclusive prefix of the tile
+        // counts of k_topo_pre; simulations with larger leaves take theirs from
+        // the column pass's look-back).
+        if (S.bucket > kLocalBucket) return;
+        uint32_t cc = 0, cl = 0;
+        for (uint32_t c = 0; c < chunks; ++c) {
+            const uint32_t i = c * 64 + lane;
+            const size_t at = (size_t)(t0 + min(i, T - 1u)) * kTopoSum + 2 * kLevels;
+            const uint32_t vc = i < T ? tsum[at] : 0u, vl = i < T ? tsum[at + 1] : 0u;
+            const uint32_t ic = wave_incl_sum_u32(vc), il = wave_incl_sum_u32(vl);
+            if (i < T) {
+                psum[at] = cc + ic - vc;
+                psum[at + 1] = cl + il - vl;
+            }
+            cc += __builtin_amdgcn_readlane(ic, 63);
+            cl += __builtin_amdgcn_readlane(il, 63);
+        }
+        return;
+    }
     const bool forward = blockIdx.y < kLevels;
     const uint32_t d = forward ? blockIdx.y : blockIdx.y - kLevels;
     uint32_t carry = forward ? 0u : S.n;  // max of "last + 1" before / min of "first" after
@@ -1249,17 +1338,30 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     __shared__ uint16_t s_list[kTT];  // column heads, then internal cells (tile offsets of their columns)
     __shared__ uint8_t s_dmin[kTT], s_count[kTT];
     __shared__ float s_T[kLevels];  // per-depth thresholds (a global load among the stores would wait for them)
-    // simulation from the block index, tile from a ticket of that simulation
-    // (as in k_sort_pass)
-    const uint32_t sim = ttiles[block].sim;  // (an XCD-contiguous order measured slower here)
+    // Leaf sizes up to kLocalBucket: the columns were counted by k_topo_pre and
+    // the tile's place in the preorder comes from k_topo_scan, so tiles are
+    // independent and the block index is the tile.  Larger leaves: the columns
+    // are counted here and the place comes from a decoupled look-back, the
+    // simulation from the block index and the tile from a ticket of that
+    // simulation (as in k_sort_pass), so a look-back only waits for tiles whose
+    // workgroups run.
+    TileRef tr = ttiles[block];
+    const uint32_t sim = tr.sim;
     const SimDesc& S = sim_at(sims, sim);
-    if (threadIdx.x == 0) s_ticket = S.first_topo_tile + atomicAdd(&S.meta->ticket[4], 1u);
-    lds_barrier();
-    const uint32_t t = min(s_ticket, grid - 1u);
-    const TileRef tr = ttiles[t];
-    if (tr.sim != sim) {  // tickets not reset: no tile of this simulation is left
-        if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
-        return;
+    const bool counted = S.bucket <= kLocalBucket;
+    uint32_t t = block;
+    if (!counted) {  // (uniform)
+        if (threadIdx.x == 0) s_ticket = atomicAdd(&S.meta->ticket[4], 1u);
+        lds_barrier();
+        // the simulation's tiles are consecutive and of kTT positions, so the
+        // ticket gives the tile without a load of the tile table
+        const uint32_t local = s_ticket;
+        if (local >= (S.n + kTT - 1u) / kTT) {  // tickets not reset: no tile of this simulation is left
+            if (threadIdx.x == 0) atomicOr(&S.meta->err, kErrLookback);
+            return;
+        }
+        t = S.first_topo_tile + local;
+        tr.base = local * kTT;
     }
     if (threadIdx.x < kLevels) s_T[threadIdx.x] = S.meta->T[threadIdx.x];
     const uint32_t n = S.n, bucket = S.bucket, maxd = S.maxd;
@@ -1267,11 +1369,13 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     const uint32_t base = tr.base;
     const uint32_t tile_end = min(base + kTT, n);
     float4 body[kPer];
+    uint32_t count[kPer], pl[kPer], mine[kPer];
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) {
         const uint32_t o = j * kThreads + threadIdx.x, p = base + o;
         s_key[o] = p < n ? K[p] : 0u;
         s_dmin[o] = p < n ? S.cdmin[p] : (uint8_t)17u;
+        count[j] = counted && p < n ? S.ccount[p] : 0u;
         body[j] = S.sbody[min(p, n - 1u)];
         s_body[o] = body[j];
     }
@@ -1279,15 +1383,17 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
     lds_barrier();
     tb.build(base, n, s_dmin);
     auto key_at = [&](uint32_t q) { return q - base < tile_end - base ? s_key[q - base] : K[q]; };
-    // columns: existence from the parent group's size, leaf depth from the key at p + bucket
-    uint32_t count[kPer], pl[kPer], mine[kPer];
+    // columns: existence from the parent group's size, leaf depth from the key
+    // at p + bucket (or the count of k_topo_pre); the parent group's start
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) {
         const uint32_t o = j * kThreads + threadIdx.x, p = base + o;
-        count[j] = 0;
         pl[j] = 0;
         const uint32_t dmin = s_dmin[o];
-        if (p < n && dmin <= maxd) {
+        if (counted) {
+            if (count[j] > 0u && dmin >= 1u) pl[j] = tb.prev(base, p, dmin - 1u);
+        } else if (p < n && dmin <= maxd) {
+            count[j] = 0;
             const uint32_t kp = s_key[o];
             bool exists = true;
             if (dmin >= 1) {
@@ -1315,7 +1421,18 @@ __device__ __forceinline__ void topo_count_body(const SimDesc* __restrict__ sims
         s_mine[j * kThreads + threadIdx.x] = mine[j];
         total += part;
     }
-    if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
+    if (counted) {
+        if (threadIdx.x == 0) {
+            s_excl = psum[(size_t)t * kTopoSum + 2 * kLevels];
+            s_excl_cols = psum[(size_t)t * kTopoSum + 2 * kLevels + 1];
+            if (base + kTT >= n) {
+                const uint32_t ncells = s_excl + (total & 0xffffu);
+                const uint32_t ncols = s_excl_cols + (total >> 16);
+                S.meta->ncells = ncells;
+                S.meta->nnodes = ncells - ncols + n;  // internal cells + bodies
+            }
+        }
+    } else if (threadIdx.x < 64) {  // wavefront 0 publishes and looks back
         unsigned long long* f = tflags + t;
         const unsigned long long tot = pack2(total & 0xffffu, total >> 16);
         unsigned long long excl;
@@ -1802,6 +1919,7 @@ __device__ __forceinline__ void topo_single_body(const SimDesc* __restrict__ sim
         psum[threadIdx.x] = 0u;
         psum[kLevels + threadIdx.x] = S.n;
     }
+    if (threadIdx.x < 2) psum[2 * kLevels + threadIdx.x] = 0u;  // no cells or columns before the tile
     __syncthreads();
     topo_count_body<kTopoTile>(sims, ttiles, ctl, tflags, psum, 0u, 1u);
     __syncthreads();

@@ -390,7 +390,7 @@ struct SimSet {
         const dim3 grid((uint32_t)ttopo.size());
         hipLaunchKernelGGL((k_topo_pre<kTT, kPT>), grid, dim3(kPT), 0, s, d_desc, d_ttopo, d_tsum);
         HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size(), 2 * kLevels), dim3(64), 0, s, d_desc, d_tsum,
+        hipLaunchKernelGGL(k_topo_scan, dim3((uint32_t)desc.size(), 2 * kLevels + 1), dim3(64), 0, s, d_desc, d_tsum,
                            d_psum, kTT);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL((k_topo_count<kTT, kPT>), grid, dim3(kPT), 0, s, d_desc, d_ttopo, d_ctl, d_tflags,
