@@ -1585,9 +1585,10 @@ __device__ __forceinline__ Agg combine_children_if(const SimDesc& S, uint32_t id
 // word is still to be computed from the skip target.
 __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, uint32_t ni, uint32_t p, uint32_t d,
                                                 bool inside, const Agg& a, uint32_t e, uint32_t w,
-                                                const uint32_t* lds_ch, uint32_t slot = kNone) {
+                                                const uint32_t* lds_ch, uint32_t slot = kNone,
+                                                bool span_node = true) {
     const float T = S.meta->T[d];
-    if (w == 0u) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d, S.wlayout);
+    if (w == 0u && (inside || span_node)) w = node_word((e < S.n ? S.nbase[e] : S.meta->nnodes) - ni, d, S.wlayout);
     if (inside) {
         if (lds_ch) S.cchild[idx] = make_uint4(lds_ch[0], lds_ch[1], lds_ch[2], lds_ch[3]);
         S.cagg[idx] = a;
@@ -1602,9 +1603,10 @@ __device__ __forceinline__ void link_write_cell(const SimDesc& S, uint32_t idx, 
         } else {
             atomicOr(&S.meta->err, kErrSpanCapacity);
         }
-        // aggregates follow in k_agg_top; the cell's other fields are final
+        // aggregates follow in k_agg_top (which writes the whole node); the
+        // cell's other fields are final
         S.cells[idx] = CellT{0.f, 0.f, 0.f, T, p, e, 0u, info};
-        S.nodes[ni] = NodeHot{0.f, 0.f, 0.f, w};
+        if (span_node) S.nodes[ni] = NodeHot{0.f, 0.f, 0.f, w};
         if (lds_ch) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -1734,7 +1736,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     __shared__ uint8_t s_pdm[kTT];
     __shared__ uint32_t s_dcnt[kLevels], s_doff[kLevels + 1];
     __shared__ float s_T[kLevels];
-    __shared__ uint32_t s_cb0, s_cb1;
+    __shared__ uint32_t s_cb0, s_cb1, s_nbend, s_maxb;
     const TileRef tr = ttiles[xcd_tile(block, grid)];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
@@ -1750,7 +1752,10 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         base = S.cbase[p];
         nb = S.nbase[p];
     }
-    if (o == 0) s_cb0 = base;  // cells before the tile
+    if (o == 0) {
+        s_cb0 = base;  // cells before the tile
+        s_nbend = tile_end < n ? S.nbase[tile_end] : S.meta->nnodes;  // node index after the tile
+    }
     if (p == last) s_cb1 = base + count;
     if (o < kLevels) {
         s_dcnt[o] = 0u;
@@ -1796,6 +1801,8 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         const uint32_t c = o < kLevels ? s_dcnt[o] : 0u;
         const uint32_t incl = wave_incl_sum_u32(c);
         if (o <= kLevels) s_doff[o] = incl - c;
+        const uint32_t mx = wave_incl_max(c);
+        if (o == 63) s_maxb = mx;  // the largest bucket
     }
     lds_barrier();
     if (count > 0u) {
@@ -1821,46 +1828,65 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
     lds_barrier();
     const uint32_t nint = s_doff[kLevels];
-    // Internal cells, one per thread: range end, skip target, node word, cold
-    // record and the first child (the same column, depth d + 1).
+    // Internal cells, one per thread: range end, node word, cold record and the
+    // first child (the same column, depth d + 1).  A cell inside the tile skips
+    // to the node of a position in the tile (or the one after it); a
+    // tile-crossing cell's node is written whole by k_agg_top.
+    const uint32_t nbend = s_nbend;
     for (uint32_t j = o; j < nint; j += kTT) {
         const uint32_t ent = s_bcol[j];
         const uint32_t oc = ent >> 5, k = ent & 31u;
         const uint32_t r = s_blist[j];
         const uint32_t idx = cb0 + r, ni = s_pnb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
         const uint32_t e = S.cend[idx];
-        const uint32_t skip = e < n ? S.nbase[e] : nnodes;
+        const uint32_t skip = e < min(tile_end, n) ? s_pnb[e - tr.base] : nbend;
         S.ncold[ni] = NodeCold{pc, e - pc, s_T[d], d};
-        s_w[r] = node_word(skip - ni, d, S.wlayout);  // internal depths are 0..15
+        s_w[r] = node_word(skip - ni, d, S.wlayout);  // internal depths are 0..15 (kept for cells inside the tile)
         s_e[r] = e;
         s_ch[r][(s_pkey[oc] >> (30u - 2u * d)) & 3u] = (uint16_t)(r + 1u);
     }
     lds_barrier();
     // Bottom-up rounds, deepest depth first, over the cells of each depth that
     // lie inside the tile (fill_aggregates internal branch, flat.rs:196-213):
-    // LDS traffic only, so the barriers need not wait for global stores.
-    for (int d = 15; d >= 0; --d) {
-        const uint32_t lo = s_doff[d], hi = s_doff[d + 1];
-        if (lo == hi) continue;  // (uniform) no internal cell of this depth
-        for (uint32_t j = lo + o; j < hi; j += kTT) {
-            const uint32_t r = s_blist[j];
-            if (s_e[r] > tile_end) continue;  // spanning: k_agg_top
-            double mass = 0.0, wx = 0.0, wy = 0.0;
+    // LDS traffic only.  When no depth has more than 64 internal cells,
+    // wavefront 0 runs every round alone: a wavefront's LDS accesses take
+    // effect in order, so the rounds need no workgroup barriers.
+    auto round_cell = [&](uint32_t j) {
+        const uint32_t r = s_blist[j];
+        if (s_e[r] > tile_end) return;  // spanning: k_agg_top
+        double mass = 0.0, wx = 0.0, wy = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t c = s_ch[r][q];
-                if (c != kNoLocal) {
-                    const double cm = s_m[c];
-                    mass = mass + cm;
-                    wx = wx + cm * s_x[c];
-                    wy = wy + cm * s_y[c];
-                }
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t c = s_ch[r][q];
+            if (c != kNoLocal) {
+                const double cm = s_m[c];
+                mass = mass + cm;
+                wx = wx + cm * s_x[c];
+                wy = wy + cm * s_y[c];
             }
-            s_m[r] = mass;
-            s_x[r] = wx / mass;
-            s_y[r] = wy / mass;
+        }
+        s_m[r] = mass;
+        s_x[r] = wx / mass;
+        s_y[r] = wy / mass;
+    };
+    if (s_maxb <= 64u) {  // (uniform)
+        if (o < 64u) {
+            for (int d = 15; d >= 0; --d) {
+                const uint32_t lo = s_doff[d], hi = s_doff[d + 1];
+                if (lo == hi) continue;  // (uniform) no internal cell of this depth
+                if (lo + o < hi) round_cell(lo + o);
+                // (the next round's loads stay behind this round's stores)
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            }
         }
         lds_barrier();
+    } else {
+        for (int d = 15; d >= 0; --d) {
+            const uint32_t lo = s_doff[d], hi = s_doff[d + 1];
+            if (lo == hi) continue;  // (uniform) no internal cell of this depth
+            for (uint32_t j = lo + o; j < hi; j += kTT) round_cell(j);
+            lds_barrier();
+        }
     }
     // Records, one internal cell per thread.  A tile-crossing cell takes a span
     // slot (one atomic per wavefront) and the links to its children that lie in
@@ -1892,7 +1918,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         for (int q = 0; q < 4; ++q) ch[q] = s_ch[r][q] == kNoLocal ? kNoChild : cb0 + s_ch[r][q];
         const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
         link_write_cell(S, cb0 + r, s_pnb[oc] + k, tr.base + oc, s_pdm[oc] + k, in, a, e, s_w[r], ch,
-                        in ? kNone : slot);
+                        in ? kNone : slot, false);
     }
 }
 
@@ -2170,15 +2196,18 @@ __device__ __forceinline__ Agg agg_span_global(const SimDesc& S, uint32_t idx) {
     return sum_children(c, m, x, y);
 }
 
-__device__ __forceinline__ void store_span(const SimDesc& S, uint32_t idx, uint32_t node, const Agg& a) {
+// A spanning cell's aggregate and its whole node (w: the node word).
+__device__ __forceinline__ void store_span(const SimDesc& S, uint32_t idx, uint32_t node, const Agg& a, uint32_t w) {
     S.cagg[idx] = a;
     S.cells[idx].comx = (float)a.x;
     S.cells[idx].comy = (float)a.y;
     S.cells[idx].mass = (float)a.m;
-    NodeHot* nd = &S.nodes[node];
-    nd->x = (float)a.x;
-    nd->y = (float)a.y;
-    nd->m = (float)a.m;
+    S.nodes[node] = NodeHot{(float)a.x, (float)a.y, (float)a.m, w};
+}
+
+// Node word of a spanning cell: its subtree runs to the node of its range end.
+__device__ __forceinline__ uint32_t span_word(const SimDesc& S, uint32_t end, uint32_t node, uint32_t depth) {
+    return node_word((end < S.n ? S.nbase[end] : S.meta->nnodes) - node, depth, S.wlayout);
 }
 
 // Also gathers the sticky error words as k_collect_errors does (errs[0] = the
@@ -2192,6 +2221,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     __shared__ double s_m[kTopLds], s_x[kTopLds], s_y[kTopLds];
     __shared__ uint4 s_ref[kTopLdsCells];
     __shared__ uint32_t s_idx[kTopLdsCells], s_node[kTopLdsCells];  // cell and node index
+    __shared__ uint32_t s_w[kTopLdsCells];                           // node word
     __shared__ uint16_t s_list[kTopLdsCells];                        // slots bucketed by depth
     __shared__ uint8_t s_depth[kTopLdsCells];
     __shared__ uint32_t s_next, s_cnt[kLevels], s_off[kLevels + 1];
@@ -2217,6 +2247,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             s_depth[i] = (uint8_t)depth;
             atomicAdd(&s_cnt[depth], 1u);
             const uint4 ch = S.cchild[idx];
+            const uint32_t end = S.cells[idx].end;
             uint32_t c[4], info[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -2226,6 +2257,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             Agg ca[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) ca[q] = S.cagg[c[q] == kNoChild ? idx : c[q]];
+            s_w[i] = span_word(S, end, se.y & kSpanNodeMask, depth);
             uint4 ref;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -2286,7 +2318,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
                 s_m[i] = a.m;
                 s_x[i] = a.x;
                 s_y[i] = a.y;
-                store_span(S, s_idx[i], s_node[i], a);
+                store_span(S, s_idx[i], s_node[i], a, s_w[i]);
             }
             // the next depth reads only LDS: no wait for this depth's global stores
             lds_barrier();
@@ -2317,7 +2349,8 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         if (s_off[d] == s_off[d + 1]) continue;  // (uniform) no spanning cell at this depth
         for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kTopThreads) {
             const uint2 se = S.spanbuf[k];
-            store_span(S, se.x, se.y & kSpanNodeMask, agg_span_global(S, se.x));
+            const uint32_t node = se.y & kSpanNodeMask;
+            store_span(S, se.x, node, agg_span_global(S, se.x), span_word(S, S.cells[se.x].end, node, se.y >> 26));
         }
         __syncthreads();  // the next depth reads this depth's global stores
     }
