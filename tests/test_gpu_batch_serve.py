@@ -65,6 +65,32 @@ def test_batch_equals_solo(oracle, gpu, use_graph, sort_tile, block, waves, buck
     b.close()
 
 
+@pytest.mark.parametrize("topo_tile,streams", [(512, 1), (1024, 1), (1024, 2)])
+def test_batch_mixed_leaf_sizes(oracle, gpu, topo_tile, streams):
+    """Leaf sizes on both sides of 32 in one batch: simulations with small leaves
+    have their columns counted in the boundary pass and their tiles' places in the
+    preorder from the scan, those with larger leaves take a ticket and a decoupled
+    look-back in the column pass, all in the same launches, at both topology tile
+    sizes.  Every simulation follows the oracle's trajectory and evidence."""
+    sims = [(ics.rotating_disc(5000, 3001, scale=0.8), 1), (ics.uniform_collapse(3000, 3002), 48),
+            (ics.rotating_disc(2500, 3003, scale=1.5), 4), (ics.uniform_collapse(4100, 3004), 200),
+            (ics.rotating_disc(1500, 3005), 32), (ics.rotating_disc(2600, 3006, scale=0.6), 33)]
+    cfgs = [Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=bk, max_depth=16) for _, bk in sims]
+    print(f"mixed leaf sizes topo_tile={topo_tile} streams={streams}", flush=True)
+    b = Batch(gpu, [s for s, _ in sims], cfgs, 1e-3, use_graph=True, topo_tile=topo_tile, streams=streams)
+    b.step(2)
+    for i, ((s0, _), cfg) in enumerate(zip(sims, cfgs)):
+        s, a = b.read(i)
+        rs, ra = oracle.f32_evolve(s0, 2, 1e-3, ocfg(oracle, cfg))
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+        st, hsh = b.force_evidence(i)
+        t = oracle.tree_from_state(s, ocfg(oracle, cfg))
+        _, ostats, ohash = oracle.traverse_f32(*t.pack(), evidence=True)
+        assert np.array_equal(st, ostats) and np.array_equal(hsh, ohash), i
+    b.close()
+
+
 @pytest.mark.parametrize("n,use_graph", [(2, True), (700, True), (1024, False), (1025, True)])
 def test_batch_of_one_small_simulation(oracle, gpu, n, use_graph):
     """A batch of one simulation of at most 1,024 bodies builds in one launch per
