@@ -1723,20 +1723,25 @@ template <uint32_t kTT>
 __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
                                                const TileRef* __restrict__ ttiles, uint32_t block,
                                                uint32_t grid) {
-    constexpr uint32_t kLinkCells = kTT + kTT / 2;
-    static_assert(kLinkCells < kNoLocal && kTT <= 2048, "LDS cell offsets and bucket entries are 16-bit");
+    // The LDS stage holds 2 cells per position (trees with one body per leaf
+    // have about 1.7), kept under 40 KB for 512-position tiles so that four
+    // workgroups share a CU: cells are 16-bit offsets from cb0, range ends
+    // 16-bit offsets from the tile base (kSpanEnd: past the tile), node words
+    // are formed when the records are written.
+    constexpr uint32_t kLinkCells = 2 * kTT;
+    constexpr uint16_t kSpanEnd = 0xffffu;
+    static_assert(kLinkCells < kNoLocal && kTT <= 1024, "LDS cell offsets and bucket entries are 16-bit");
     // per cell of the tile (offset from cb0)
     __shared__ double s_m[kLinkCells], s_x[kLinkCells], s_y[kLinkCells];  // f64 aggregates
     __shared__ uint16_t s_ch[kLinkCells][4];                               // children (offsets)
-    __shared__ uint32_t s_w[kLinkCells], s_e[kLinkCells];                 // node word, range end
-    __shared__ uint16_t s_blist[kLinkCells];  // internal cells bucketed by depth: cell offset
-    __shared__ uint16_t s_bcol[kLinkCells];   // ... and (column offset << 5) | k
+    __shared__ uint16_t s_e[kLinkCells];     // range end - tile base (kSpanEnd: beyond the tile); first a rank
+    __shared__ uint16_t s_bcol[kLinkCells];  // internal cells bucketed by depth: (column offset << 5) | k
     // per position of the tile
-    __shared__ uint32_t s_pnb[kTT], s_pkey[kTT];
+    __shared__ uint16_t s_pcb[kTT], s_pnb[kTT];  // the column's first cell - cb0, its node - nb0
     __shared__ uint8_t s_pdm[kTT];
     __shared__ uint32_t s_dcnt[kLevels], s_doff[kLevels + 1];
     __shared__ float s_T[kLevels];
-    __shared__ uint32_t s_cb0, s_cb1, s_nbend, s_maxb;
+    __shared__ uint32_t s_cb0, s_cb1, s_nb0, s_nbend, s_maxb;
     const TileRef tr = ttiles[xcd_tile(block, grid)];
     const SimDesc& S = sim_at(sims, tr.sim);
     const uint32_t n = S.n;
@@ -1754,6 +1759,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
     if (o == 0) {
         s_cb0 = base;  // cells before the tile
+        s_nb0 = nb;    // nodes before the tile
         s_nbend = tile_end < n ? S.nbase[tile_end] : S.meta->nnodes;  // node index after the tile
     }
     if (p == last) s_cb1 = base + count;
@@ -1771,21 +1777,22 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
     const uint32_t kp = ok ? S.key0[p] : 0u;
     lds_barrier();  // (LDS only: the parent loads stay in flight until the links below)
-    const uint32_t cb0 = s_cb0, cb1 = s_cb1;
+    const uint32_t cb0 = s_cb0, cb1 = s_cb1, nb0 = s_nb0;
     const uint32_t ntile = cb1 - cb0;
     if (!(cb0 <= cb1 && ntile <= kLinkCells)) {  // (uniform) the tile's cells exceed the LDS stage
         topo_link_columns<kTT>(S, p, tile_end, ok ? count : 0u, dmin, base, nb, kp, pl_base, pl_dmin);
         return;
     }
-    const uint32_t ncells = S.meta->ncells, nnodes = S.meta->nnodes;
-    // a consistent scan puts every column of the tile inside [cb0, cb1)
-    if (ok && (base < cb0 || base + count > cb1)) {
+    const uint32_t ncells = S.meta->ncells;
+    // a consistent scan puts every column of the tile inside [cb0, cb1), and its
+    // node indices within the tile's positions and internal cells
+    if (ok && (base < cb0 || base + count > cb1 || nb < nb0 || nb - nb0 > kTT + ntile)) {
         atomicOr(&S.meta->err, kErrCellCapacity);
         ok = false;
     }
     if (!ok) count = 0;
-    s_pnb[o] = nb;
-    s_pkey[o] = kp;
+    s_pcb[o] = (uint16_t)(base - cb0);
+    s_pnb[o] = (uint16_t)(nb - nb0);
     s_pdm[o] = (uint8_t)dmin;
     for (uint32_t i = o; i < ntile; i += kTT) {
 #pragma unroll
@@ -1794,7 +1801,8 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     Agg leaf{0.0, 0.0, 0.0, 0.0};
     if (count > 0u) leaf = S.cagg[base + count - 1u];  // the leaf's aggregate (k_topo_count)
     // rank of each internal cell among the tile's cells of its depth (held in s_e until the list is built)
-    for (uint32_t k = 0; k + 1u < count; ++k) s_e[base - cb0 + k] = atomicAdd(&s_dcnt[dmin + k], 1u);
+    for (uint32_t k = 0; k + 1u < count; ++k)
+        s_e[base - cb0 + k] = (uint16_t)atomicAdd(&s_dcnt[dmin + k], 1u);
     // (the barriers from here on order LDS only: no wait for the global stores)
     lds_barrier();
     if (o < 64) {  // bucket offsets: exclusive prefix of the per-depth counts
@@ -1805,21 +1813,25 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         if (o == 63) s_maxb = mx;  // the largest bucket
     }
     lds_barrier();
+    // Per column: the leaf's aggregate, the column's internal cells into their
+    // depth buckets with the link to their first child (the same column, depth
+    // d + 1), and the link from the parent group's cell to the column head.
     if (count > 0u) {
-        const uint32_t r = base + count - 1u - cb0;
+        const uint32_t r0 = base - cb0;
+        const uint32_t r = r0 + count - 1u;
         s_m[r] = leaf.m;
         s_x[r] = leaf.x;
         s_y[r] = leaf.y;
         for (uint32_t k = 0; k + 1u < count; ++k) {
-            const uint32_t j = s_doff[dmin + k] + s_e[base - cb0 + k];
-            s_blist[j] = (uint16_t)(base - cb0 + k);
-            s_bcol[j] = (uint16_t)((o << 5) | k);
+            const uint32_t d = dmin + k;
+            s_bcol[s_doff[d] + s_e[r0 + k]] = (uint16_t)((o << 5) | k);
+            s_ch[r0 + k][(kp >> (30u - 2u * d)) & 3u] = (uint16_t)(r0 + k + 1u);
         }
         if (dmin >= 1) {
             const uint32_t parent = pl_base + (dmin - 1u - pl_dmin);
             const uint32_t q = (kp >> (30u - 2u * (dmin - 1u))) & 3u;
             if (pl_dmin <= dmin - 1u && parent < ncells && parent < S.cell_cap) {
-                if (parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = (uint16_t)(base - cb0);
+                if (parent >= cb0 && parent < cb1) s_ch[parent - cb0][q] = (uint16_t)r0;
                 else (&S.cchild[parent].x)[q] = base;  // a parent outside the tile's cells
             } else {
                 atomicOr(&S.meta->err, kErrCellCapacity);
@@ -1828,22 +1840,14 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     }
     lds_barrier();
     const uint32_t nint = s_doff[kLevels];
-    // Internal cells, one per thread: range end, node word, cold record and the
-    // first child (the same column, depth d + 1).  A cell inside the tile skips
-    // to the node of a position in the tile (or the one after it); a
-    // tile-crossing cell's node is written whole by k_agg_top.
-    const uint32_t nbend = s_nbend;
+    // Internal cells, one per thread: range end and cold record.
     for (uint32_t j = o; j < nint; j += kTT) {
         const uint32_t ent = s_bcol[j];
         const uint32_t oc = ent >> 5, k = ent & 31u;
-        const uint32_t r = s_blist[j];
-        const uint32_t idx = cb0 + r, ni = s_pnb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
-        const uint32_t e = S.cend[idx];
-        const uint32_t skip = e < min(tile_end, n) ? s_pnb[e - tr.base] : nbend;
-        S.ncold[ni] = NodeCold{pc, e - pc, s_T[d], d};
-        s_w[r] = node_word(skip - ni, d, S.wlayout);  // internal depths are 0..15 (kept for cells inside the tile)
-        s_e[r] = e;
-        s_ch[r][(s_pkey[oc] >> (30u - 2u * d)) & 3u] = (uint16_t)(r + 1u);
+        const uint32_t r = s_pcb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
+        const uint32_t e = S.cend[cb0 + r];
+        S.ncold[nb0 + s_pnb[oc] + k] = NodeCold{pc, e - pc, s_T[d], d};
+        s_e[r] = e <= tile_end ? (uint16_t)(e - tr.base) : kSpanEnd;
     }
     lds_barrier();
     // Bottom-up rounds, deepest depth first, over the cells of each depth that
@@ -1852,8 +1856,9 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
     // wavefront 0 runs every round alone: a wavefront's LDS accesses take
     // effect in order, so the rounds need no workgroup barriers.
     auto round_cell = [&](uint32_t j) {
-        const uint32_t r = s_blist[j];
-        if (s_e[r] > tile_end) return;  // spanning: k_agg_top
+        const uint32_t ent = s_bcol[j];
+        const uint32_t r = s_pcb[ent >> 5] + (ent & 31u);
+        if (s_e[r] == kSpanEnd) return;  // spanning: k_agg_top
         double mass = 0.0, wx = 0.0, wy = 0.0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1888,21 +1893,24 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
             lds_barrier();
         }
     }
-    // Records, one internal cell per thread.  A tile-crossing cell takes a span
-    // slot (one atomic per wavefront) and the links to its children that lie in
-    // the tile (those in later tiles were or will be written by their tiles).
+    // Records, one internal cell per thread.  A cell inside the tile skips to
+    // the node of a position in the tile (or the one after it).  A tile-crossing
+    // cell takes a span slot (one atomic per wavefront) and the links to its
+    // children that lie in the tile (those in later tiles were or will be
+    // written by their tiles); k_agg_top writes its aggregate and whole node.
+    const uint32_t nbend = s_nbend, lim = min(tile_end, n);
     for (uint32_t j0 = 0; j0 < nint; j0 += kTT) {  // (uniform trip count)
         const uint32_t j = j0 + o;
         const bool have = j < nint;
-        uint32_t r = 0, oc = 0, k = 0;
+        uint32_t r = 0, oc = 0, k = 0, eo = 0;
         if (have) {
             const uint32_t ent = s_bcol[j];
             oc = ent >> 5;
             k = ent & 31u;
-            r = s_blist[j];
+            r = s_pcb[oc] + k;
+            eo = s_e[r];
         }
-        const uint32_t e = have ? s_e[r] : 0u;
-        const bool in = e <= tile_end;
+        const bool in = eo != kSpanEnd;
         const unsigned long long span = __ballot(have && !in);
         uint32_t slot = kNone;
         if (span) {
@@ -1913,12 +1921,20 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
             slot = first + (uint32_t)__builtin_popcountll(span & ((1ull << lane_id()) - 1ull));
         }
         if (!have) continue;
+        const uint32_t idx = cb0 + r, ni = nb0 + s_pnb[oc] + k, d = s_pdm[oc] + k;
+        uint32_t e, w = 0;
+        if (in) {
+            e = tr.base + eo;
+            const uint32_t skip = e < lim ? nb0 + s_pnb[eo] : nbend;
+            w = node_word(skip - ni, d, S.wlayout);  // internal depths are 0..15
+        } else {
+            e = S.cend[idx];
+        }
         uint32_t ch[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) ch[q] = s_ch[r][q] == kNoLocal ? kNoChild : cb0 + s_ch[r][q];
         const Agg a{s_m[r], s_x[r], s_y[r], 0.0};
-        link_write_cell(S, cb0 + r, s_pnb[oc] + k, tr.base + oc, s_pdm[oc] + k, in, a, e, s_w[r], ch,
-                        in ? kNone : slot, false);
+        link_write_cell(S, idx, ni, tr.base + oc, d, in, a, e, w, ch, in ? kNone : slot, false);
     }
 }
 
