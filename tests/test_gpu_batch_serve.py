@@ -91,6 +91,44 @@ def test_batch_mixed_leaf_sizes(oracle, gpu, topo_tile, streams):
     b.close()
 
 
+def paired_disc(n, seed):
+    """Bodies in tight pairs (1e-5 apart): with one body per leaf every pair hangs
+    from a chain of single-child cells, about 5.4 cells per position."""
+    rng = np.random.default_rng(seed)
+    base = ics.rotating_disc(n // 2, seed)
+    s = np.zeros((2 * (n // 2), 8), np.float32)
+    s[0::2] = base
+    s[1::2] = base
+    s[1::2, 0] += np.float32(1e-5) * rng.standard_normal(n // 2).astype(np.float32)
+    s[1::2, 1] += np.float32(1e-5) * rng.standard_normal(n // 2).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("topo_tile", [512, 1024])
+def test_batch_dense_trees_take_the_global_link_path(oracle, gpu, topo_tile):
+    """Tiles with more than 2 cells per position exceed the link kernel's LDS stage
+    and run its column-per-lane global-memory rounds, beside simulations whose
+    tiles fit the stage; every simulation follows the oracle's trajectory and
+    evidence."""
+    c1 = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=1, max_depth=16)
+    c4 = Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)
+    sims = [(paired_disc(3000, 4001), c1), (ics.rotating_disc(2500, 4002), c1), (paired_disc(1800, 4003), c4),
+            (ics.rotating_disc(4000, 4004, scale=0.8), c4)]
+    print(f"dense trees topo_tile={topo_tile}", flush=True)
+    b = Batch(gpu, [s for s, _ in sims], [c for _, c in sims], 1e-3, use_graph=True, topo_tile=topo_tile)
+    b.step(2)
+    for i, (s0, cfg) in enumerate(sims):
+        s, a = b.read(i)
+        rs, ra = oracle.f32_evolve(s0, 2, 1e-3, ocfg(oracle, cfg))
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+        st, hsh = b.force_evidence(i)
+        t = oracle.tree_from_state(s, ocfg(oracle, cfg))
+        _, ostats, ohash = oracle.traverse_f32(*t.pack(), evidence=True)
+        assert np.array_equal(st, ostats) and np.array_equal(hsh, ohash), i
+    b.close()
+
+
 @pytest.mark.parametrize("n,use_graph", [(2, True), (700, True), (1024, False), (1025, True)])
 def test_batch_of_one_small_simulation(oracle, gpu, n, use_graph):
     """A batch of one simulation of at most 1,024 bodies builds in one launch per
