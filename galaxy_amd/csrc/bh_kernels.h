@@ -2180,6 +2180,7 @@ __global__ __launch_bounds__(kTopoTile) void k_build_small(const SimDesc* __rest
 constexpr uint32_t kTopThreads = 256;
 constexpr uint32_t kTopLdsCells = 256;      // spanning cells aggregated in LDS (sets of several simulations)
 constexpr uint32_t kTopLdsCellsOne = 768;   // the same for a set of one simulation (nothing runs beside it)
+constexpr uint32_t kTopThreadsOne = 1024;   // threads for a set of one simulation
 constexpr uint32_t kSpanNodeMask = (1u << 26) - 1u;  // span entry: (cell, depth << 26 | node)
 
 // Sum of present children in quadrant order (c[q] == kNoChild: absent).
@@ -2229,8 +2230,8 @@ __device__ __forceinline__ uint32_t span_word(const SimDesc& S, uint32_t end, ui
 // Also gathers the sticky error words as k_collect_errors does (errs[0] = the
 // set's, errs[1 + s] = simulation s's): it runs last in a build, and no kernel
 // after it in a step sets them.
-template <uint32_t kTopLdsCells>
-__global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restrict__ sims,
+template <uint32_t kTopLdsCells, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_agg_top(const SimDesc* __restrict__ sims,
                                                          const uint32_t* __restrict__ ctl,
                                                          uint32_t* __restrict__ errs) {
     constexpr uint32_t kTopLds = 5 * kTopLdsCells;  // the cells' slots, then preloaded children
@@ -2253,7 +2254,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     if (threadIdx.x == 0) s_next = nspan;
     __syncthreads();
     if (nspan <= kTopLdsCells) {
-        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+        for (uint32_t i = threadIdx.x; i < nspan; i += kThreads) {
             // three levels of gathers: the span entry, the cell's children, then
             // the children's span slots and aggregates
             const uint2 se = S.span[i];
@@ -2309,7 +2310,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
             s_off[kLevels] = acc;
         }
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+        for (uint32_t i = threadIdx.x; i < nspan; i += kThreads) {
             const uint32_t d = s_depth[i];
             s_list[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = (uint16_t)i;
         }
@@ -2317,7 +2318,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         for (int d = 15; d >= 0; --d) {
             const uint32_t lo = s_off[d], hi = s_off[d + 1];
             if (lo == hi) continue;  // (uniform) no spanning cell at this depth
-            for (uint32_t k = lo + threadIdx.x; k < hi; k += kTopThreads) {
+            for (uint32_t k = lo + threadIdx.x; k < hi; k += kThreads) {
                 const uint32_t i = s_list[k];
                 const uint4 ref = s_ref[i];
                 uint32_t c[4];
@@ -2342,7 +2343,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         return;
     }
     // listed mode: counting sort of the span entries by depth into spanbuf
-    for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads)
+    for (uint32_t i = threadIdx.x; i < nspan; i += kThreads)
         atomicAdd(&s_cnt[min(S.span[i].y >> 26, kLevels - 1u)], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -2355,7 +2356,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
         s_off[kLevels] = acc;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nspan; i += kTopThreads) {
+    for (uint32_t i = threadIdx.x; i < nspan; i += kThreads) {
         const uint2 se = S.span[i];
         const uint32_t d = min(se.y >> 26, kLevels - 1u);
         S.spanbuf[s_off[d] + atomicAdd(&s_cnt[d], 1u)] = se;
@@ -2363,7 +2364,7 @@ __global__ __launch_bounds__(kTopThreads) void k_agg_top(const SimDesc* __restri
     __syncthreads();  // (also orders the list's global stores before the rounds' loads)
     for (int d = 15; d >= 0; --d) {
         if (s_off[d] == s_off[d + 1]) continue;  // (uniform) no spanning cell at this depth
-        for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kTopThreads) {
+        for (uint32_t k = s_off[d] + threadIdx.x; k < s_off[d + 1]; k += kThreads) {
             const uint2 se = S.spanbuf[k];
             const uint32_t node = se.y & kSpanNodeMask;
             store_span(S, se.x, node, agg_span_global(S, se.x), span_word(S, S.cells[se.x].end, node, se.y >> 26));

@@ -404,9 +404,10 @@ struct SimSet {
         // a set of one simulation stages more spanning cells in LDS (the small
         // footprint matters only beside another group's kernels)
         if (desc.size() == 1)
-            hipLaunchKernelGGL(k_agg_top<kTopLdsCellsOne>, dim3(1), dim3(kTopThreads), 0, s, d_desc, d_ctl, d_errs);
+            hipLaunchKernelGGL((k_agg_top<kTopLdsCellsOne, kTopThreadsOne>), dim3(1), dim3(kTopThreadsOne), 0, s, d_desc, d_ctl,
+                               d_errs);
         else
-            hipLaunchKernelGGL(k_agg_top<kTopLdsCells>, dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s,
+            hipLaunchKernelGGL((k_agg_top<kTopLdsCells, kTopThreads>), dim3((uint32_t)desc.size()), dim3(kTopThreads), 0, s,
                                d_desc, d_ctl, d_errs);
         HIP_OK(hipGetLastError());
     }
