@@ -128,6 +128,7 @@ struct SimMeta {
     uint32_t theta_bits;    // f32 theta the thresholds were computed for
     uint32_t pow4;          // T[d] = T[0] * 4^-d exactly for d = 1..15 (see pow4_thresholds)
     uint32_t t0_bits;       // bit pattern of T[0]
+    uint32_t contain_rejected;  // the opening test rejects every cell of depth 1..15 holding the target
     uint32_t hist[4 * 256]; // per-pass digit histograms of the Morton keys
     // tile tickets of the simulation: radix passes 0..3, topology (4)
     uint32_t ticket[8];
@@ -451,6 +452,24 @@ __device__ __forceinline__ uint32_t pow4_thresholds(float Td, float T0) {
     if (d < (int)kCodes) {
         const bool normal = T0 >= 1.17549435e-38f && T0 < __builtin_inff();
         ok = normal && (d == 0 || (Td == __builtin_ldexpf(T0, -2 * d) && __builtin_ldexpf(Td, 2 * d) == T0));
+    }
+    return __ballot(!ok) == 0ull ? 1u : 0u;
+}
+
+// Whether the opening test alone rejects every internal cell of depth 1..15 for
+// the targets the cell holds, so that a walk needs no containment test below
+// the root.  A target in the cell and the cell's centre of mass both lie in the
+// cell's box of side w = 2 half (the f32 centre of mass rounds into the range
+// of its bodies' coordinates), so d2 <= 2 w^2 up to a few relative roundings
+// of 2^-24, and d2 < T[d] whenever T[d] >= 2.25 w^2 (theta <= 2/3).  Called by
+// a whole wavefront whose lane d < kCodes holds T[d] and the depth's f32
+// half-width; returns 1 when it holds for every d = 1..15.
+__device__ __forceinline__ uint32_t contained_rejected(float Td, float half_d) {
+    const uint32_t d = lane_id();
+    bool ok = true;
+    if (d >= 1u && d < kCodes) {
+        const double w = 2.0 * (double)half_d;
+        ok = (double)Td >= 2.25 * w * w;  // (false for a NaN threshold)
     }
     return __ballot(!ok) == 0ull ? 1u : 0u;
 }
@@ -784,20 +803,23 @@ __global__ __launch_bounds__(kSortThreads) void k_keys(const SimDesc* __restrict
             // centres of mass lie in the root box), so d2 stays finite
             M->geom_ok = half < 1.0e18 ? 1u : 0u;
         }
-        float T = 0.0f;
+        float T = 0.0f, hf = 0.0f;
         if (threadIdx.x < kLevels) {
             // half at depth d is half * 2^-d exactly in f64; one rounding to f32
             double hd = half;
             for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-            T = accept_threshold((float)hd, S.theta);
+            hf = (float)hd;
+            T = accept_threshold(hf, S.theta);
             M->T[threadIdx.x] = T;
             if (threadIdx.x < kCodes) M->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
         }
         const float T0 = __shfl(T, 0);
         const uint32_t p4 = pow4_thresholds(T, T0);
+        const uint32_t cr = contained_rejected(T, hf);
         if (threadIdx.x == 0) {
             M->pow4 = p4;
             M->t0_bits = __float_as_uint(T0);
+            M->contain_rejected = cr;
         }
     }
     // Morton keys of this tile
@@ -2091,19 +2113,22 @@ __global__ __launch_bounds__(kTopoTile) void k_build_small(const SimDesc* __rest
             M->mode = 0u;
             M->geom_ok = half < 1.0e18 ? 1u : 0u;
         }
-        float T = 0.0f;
+        float T = 0.0f, hf = 0.0f;
         if (i < kLevels) {
             double hd = half;
             for (uint32_t l = 0; l < i; ++l) hd = hd * 0.5;
-            T = accept_threshold((float)hd, S.theta);
+            hf = (float)hd;
+            T = accept_threshold(hf, S.theta);
             M->T[i] = T;
             if (i < kCodes) M->tab[i] = i == 0 ? 0.0f : T;
         }
         const float T0 = __shfl(T, 0);
         const uint32_t p4 = pow4_thresholds(T, T0);
+        const uint32_t cr = contained_rejected(T, hf);
         if (i == 0) {
             M->pow4 = p4;
             M->t0_bits = __float_as_uint(T0);
+            M->contain_rejected = cr;
         }
     }
     // Morton keys, then four stable 8-bit passes: element k of the sequence is
@@ -2378,20 +2403,23 @@ __global__ void k_set_theta(const SimDesc* __restrict__ sims, uint32_t nsims) {
     const uint32_t s = blockIdx.x;
     if (s >= nsims) return;
     const SimDesc& S = sim_at(sims, s);
-    float T = 0.0f;
+    float T = 0.0f, hf = 0.0f;
     if (threadIdx.x < kLevels) {
         double hd = S.meta->root[2];
         for (uint32_t l = 0; l < threadIdx.x; ++l) hd = hd * 0.5;
-        T = accept_threshold((float)hd, S.theta);
+        hf = (float)hd;
+        T = accept_threshold(hf, S.theta);
         S.meta->T[threadIdx.x] = T;
         if (threadIdx.x < kCodes) S.meta->tab[threadIdx.x] = threadIdx.x == 0 ? 0.0f : T;
     }
     const float T0 = __shfl(T, 0);
     const uint32_t p4 = pow4_thresholds(T, T0);  // (one wavefront: blockDim.x == 64)
+    const uint32_t cr = contained_rejected(T, hf);
     if (threadIdx.x == 0) {
         S.meta->theta_bits = __float_as_uint(S.theta);
         S.meta->pow4 = p4;
         S.meta->t0_bits = __float_as_uint(T0);
+        S.meta->contain_rejected = cr;
     }
 }
 
@@ -2578,7 +2606,11 @@ struct NodeWalk {
 // where w << 22 is the code's exponent offset (node_word).  As in the fast
 // NodeWalk, the end test runs once per two steps (zero record).
 // kUnitG: G == 1, where fl(G m) = m for every m, so the product is left out.
-template <uint32_t K, bool kUnitG>
+// kContain = false: no containment test (SimMeta::contain_rejected and a walk
+// that starts below the root, see self_terms_vanish): the opening test rejects
+// every cell holding the target, and the target's own body node is accepted,
+// which adds exactly +0 to its sums.
+template <uint32_t K, bool kUnitG, bool kContain = true>
 struct MultiWalk {
     __amdgpu_buffer_rsrc_t nh;
     uint32_t nn, t0_bits;
@@ -2615,7 +2647,8 @@ struct MultiWalk {
             d2[k] = dd + ee;
             // (bitwise, not short-circuit: straight-line selects, no branches)
             const bool at = K == 1 || cur[k] == c;
-            acc[k] = at & ((self[k] - c) > size_m1) & ((int)__float_as_uint(d2[k]) >= tnode);
+            acc[k] = at & ((int)__float_as_uint(d2[k]) >= tnode);
+            if (kContain) acc[k] = acc[k] & ((self[k] - c) > size_m1);
             cur[k] = max(cur[k], acc[k] ? skip : c1);
             next = min(next, cur[k]);  // greater than c
         }
@@ -2638,11 +2671,40 @@ struct MultiWalk {
     }
 };
 
-// The multi-target walk of one work item (targets base .. base + 64 K - 1).
+// Whether every target of the item (targets base .. base + 64 K - 1) would add
+// exactly +0 to its own sums by interacting with its own body node: there dx =
+// dy = +0 (the node holds the target's own coordinates), r2 = eps2, and the
+// term is +0 * scale = +0 when scale = G m / r^3, rounded as point_mass_pos
+// rounds it, is finite (a sum that starts at +0 is never -0, so adding +0
+// leaves it unchanged).  Called by whole wavefronts.
 template <uint32_t K, bool kUnitG>
+__device__ __forceinline__ bool self_terms_vanish(const SimDesc& S, uint32_t base, uint32_t n, float eps2,
+                                                  float G) {
+    const float inv_r = rcp_rn_normal(sqrt_rn_pos(eps2));
+    bool ok = true;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t q = base + K * lane_id() + k;
+        if (q < n) {
+            const float m = S.sbody[q].z;
+            float scale = kUnitG ? m : G * m;
+            scale = scale * inv_r;
+            scale = scale * inv_r;
+            scale = scale * inv_r;
+            ok = ok && __builtin_fabsf(scale) < __builtin_inff();
+        }
+    }
+    return __ballot(!ok) == 0ull;
+}
+
+// The multi-target walk of one work item (targets base .. base + 64 K - 1),
+// starting at node `start`: 0, or 1 when the walk has no containment test and
+// node 0 is the root (the root holds every target and is always opened).
+template <uint32_t K, bool kUnitG, bool kContain = true>
 __device__ __forceinline__ void multi_walk(const SimDesc& S, uint32_t base, uint32_t n, __amdgpu_buffer_rsrc_t nh,
-                                           uint32_t nn, uint32_t t0_bits, float eps2, float G) {
-    MultiWalk<K, kUnitG> wk;
+                                           uint32_t nn, uint32_t t0_bits, float eps2, float G,
+                                           uint32_t start = 0u) {
+    MultiWalk<K, kUnitG, kContain> wk;
     wk.nh = nh;
     wk.nn = nn;
     wk.t0_bits = t0_bits;
@@ -2655,8 +2717,8 @@ __device__ __forceinline__ void multi_walk(const SimDesc& S, uint32_t base, uint
         const float4 b = S.sbody[q];
         wk.tx[k] = b.x;
         wk.ty[k] = b.y;
-        wk.self[k] = S.bnode[q];
-        wk.cur[k] = p0 + k < n ? 0u : nn;
+        if (kContain) wk.self[k] = S.bnode[q];
+        wk.cur[k] = p0 + k < n ? start : nn;
     }
     wk.run();
     __builtin_amdgcn_wave_barrier();
@@ -2758,12 +2820,36 @@ __global__ __launch_bounds__(256) void k_traverse_nodes(const SimDesc* __restric
         const bool fast = geom_ok != 0u && eps2 > 0.0f && eps2 <= 3.4028234663852886e38f;
         const __amdgpu_buffer_rsrc_t nh = node_buffer(S.nodes, nn);
         if (!kEvidence && !general && fast && pow4 && narrow != 0u) {
+            // no containment test where the thresholds reject every cell that
+            // holds its target and the item's self terms vanish; the walk then
+            // starts below the root (node 0 when the root is a leaf: then every
+            // node is a body)
+            const uint32_t cr = __builtin_amdgcn_readfirstlane(S.meta->contain_rejected);
+            const uint32_t start = __builtin_amdgcn_readfirstlane(S.meta->nnodes) > n ? 1u : 0u;
             if (G == 1.0f) {
-                if (lane_targets == 2u) multi_walk<2, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
-                else multi_walk<1, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                if (lane_targets == 2u) {
+                    if (cr && self_terms_vanish<2, true>(S, tr.base, n, eps2, G))
+                        multi_walk<2, true, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G, start);
+                    else
+                        multi_walk<2, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                } else {
+                    if (cr && self_terms_vanish<1, true>(S, tr.base, n, eps2, G))
+                        multi_walk<1, true, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G, start);
+                    else
+                        multi_walk<1, true>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                }
             } else {
-                if (lane_targets == 2u) multi_walk<2, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
-                else multi_walk<1, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                if (lane_targets == 2u) {
+                    if (cr && self_terms_vanish<2, false>(S, tr.base, n, eps2, G))
+                        multi_walk<2, false, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G, start);
+                    else
+                        multi_walk<2, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                } else {
+                    if (cr && self_terms_vanish<1, false>(S, tr.base, n, eps2, G))
+                        multi_walk<1, false, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G, start);
+                    else
+                        multi_walk<1, false>(S, tr.base, n, nh, nn, t0_bits, eps2, G);
+                }
             }
         } else {
             // this wavefront's copy of the code table (LDS is in order per wavefront)

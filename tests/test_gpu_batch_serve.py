@@ -91,6 +91,31 @@ def test_batch_mixed_leaf_sizes(oracle, gpu, topo_tile, streams):
     b.close()
 
 
+def test_batch_two_target_walks_with_and_without_containment_test(oracle, gpu):
+    """One persistent wavefront per CU, so that the set has items enough for two
+    targets per lane.  Simulations whose thresholds reject every cell holding its
+    target (theta <= 2/3) and whose self terms vanish walk without a containment
+    test (with G = 1 and G = 2); theta 0.75, and a softening of 1e-15 whose self
+    term m / eps^3 overflows, keep the test.  Every trajectory equals the
+    oracle's."""
+    sims = [(ics.rotating_disc(30000, 4001), Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)),
+            (ics.rotating_disc(30000, 4002, scale=1.4),
+             Config(theta=0.5, softening_kpc=0.01, g=2.0, bucket=4, max_depth=16)),
+            (ics.uniform_collapse(20000, 4003), Config(theta=0.75, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)),
+            (ics.rotating_disc(20000, 4004, scale=0.8),
+             Config(theta=0.5, softening_kpc=1e-15, g=1.0, bucket=4, max_depth=16))]
+    print("two-target walks: theta 0.5 (G 1, G 2), theta 0.75, softening 1e-15", flush=True)
+    b = Batch(gpu, [s for s, _ in sims], [c for _, c in sims], 1e-3, use_graph=True, traverse_block=64,
+              traverse_waves_per_cu=1)
+    b.step(2)
+    for i, (s0, cfg) in enumerate(sims):
+        s, a = b.read(i)
+        rs, ra = oracle.f32_evolve(s0, 2, 1e-3, ocfg(oracle, cfg))
+        assert np.array_equal(s.view(np.uint32), rs.view(np.uint32)), i
+        assert np.array_equal(a.view(np.uint32), ra.view(np.uint32)), i
+    b.close()
+
+
 def paired_disc(n, seed):
     """Bodies in tight pairs (1e-5 apart): with one body per leaf every pair hangs
     from a chain of single-child cells, about 5.4 cells per position."""
