@@ -95,16 +95,18 @@ def test_batch_two_target_walks_with_and_without_containment_test(oracle, gpu):
     """One persistent wavefront per CU, so that the set has items enough for two
     targets per lane.  Simulations whose thresholds reject every cell holding its
     target (theta <= 2/3) and whose self terms vanish walk without a containment
-    test (with G = 1 and G = 2); theta 0.75, and a softening of 1e-15 whose self
-    term m / eps^3 overflows, keep the test.  Every trajectory equals the
-    oracle's."""
+    test (with G = 1 and G = 2, and at theta 0.66, where the margin between a
+    containing cell's d2 <= 2 w^2 and its threshold is smallest); theta 0.75, and a
+    softening of 1e-15 whose self term m / eps^3 overflows, keep the test.  Every
+    trajectory equals the oracle's."""
     sims = [(ics.rotating_disc(30000, 4001), Config(theta=0.5, softening_kpc=0.01, g=1.0, bucket=4, max_depth=16)),
             (ics.rotating_disc(30000, 4002, scale=1.4),
              Config(theta=0.5, softening_kpc=0.01, g=2.0, bucket=4, max_depth=16)),
             (ics.uniform_collapse(20000, 4003), Config(theta=0.75, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)),
+            (ics.uniform_collapse(20000, 4005), Config(theta=0.66, softening_kpc=0.005, g=1.0, bucket=4, max_depth=16)),
             (ics.rotating_disc(20000, 4004, scale=0.8),
              Config(theta=0.5, softening_kpc=1e-15, g=1.0, bucket=4, max_depth=16))]
-    print("two-target walks: theta 0.5 (G 1, G 2), theta 0.75, softening 1e-15", flush=True)
+    print("two-target walks: theta 0.5 (G 1, G 2), theta 0.75, theta 0.66, softening 1e-15", flush=True)
     b = Batch(gpu, [s for s, _ in sims], [c for _, c in sims], 1e-3, use_graph=True, traverse_block=64,
               traverse_waves_per_cu=1)
     b.step(2)
