@@ -140,7 +140,11 @@ struct __attribute__((aligned(16))) SimDesc {
     uint32_t first_tile, first_sort_tile, first_topo_tile, span_cap;
     uint32_t node_cap;  // node records allocated (the traversal reads at most index node_cap - 1)
     uint32_t wlayout;   // hot-word layout (node_word): 1 when node_cap <= kNarrowNodes (device trees)
-    uint32_t pad_[2];
+    // 1 (batched and served sets): a step leaves the internal cells' cold records,
+    // which only an evidence traversal reads, unwritten; they are filled when one
+    // asks (k_fill_cell_cold)
+    uint32_t lean;
+    uint32_t pad_;
     // resident state (SoA)
     float2* pos;
     float* mass;
@@ -1708,7 +1712,7 @@ __device__ __forceinline__ void topo_link_columns(const SimDesc& S, uint32_t p, 
                 const uint32_t k = k0 + j;
                 if (k + 1u >= count) break;
                 const uint32_t e = ev[j];
-                S.ncold[nb + k] = NodeCold{p, e - p, tv[j], dmin + k};
+                if (!S.lean) S.ncold[nb + k] = NodeCold{p, e - p, tv[j], dmin + k};
                 if (e <= tile_end) inside |= 1u << k;
             }
         }
@@ -1868,7 +1872,7 @@ __device__ __forceinline__ void topo_link_body(const SimDesc* __restrict__ sims,
         const uint32_t oc = ent >> 5, k = ent & 31u;
         const uint32_t r = s_pcb[oc] + k, d = s_pdm[oc] + k, pc = tr.base + oc;
         const uint32_t e = S.cend[cb0 + r];
-        S.ncold[nb0 + s_pnb[oc] + k] = NodeCold{pc, e - pc, s_T[d], d};
+        if (!S.lean) S.ncold[nb0 + s_pnb[oc] + k] = NodeCold{pc, e - pc, s_T[d], d};
         s_e[r] = e <= tile_end ? (uint16_t)(e - tr.base) : kSpanEnd;
     }
     lds_barrier();
@@ -2453,6 +2457,23 @@ __global__ __launch_bounds__(kTile) void k_fill_body_cold(const SimDesc* __restr
     const uint32_t ni = S.bnode[p];
     if (ni < S.node_cap)
         S.ncold[ni] = NodeCold{p, 1u, 0.0f, kTagBody | (S.ccount[p] > 0u ? kTagFirst : 0u) | (S.val0[p] & kTagValue)};
+}
+
+// Cold records of the internal cells of a lean build, which its steps do not
+// write (only the evidence traversal reads them): from each internal cell's
+// record, at its node (the node of the cell at depth d of the column at p is
+// nbase[p] + d - cdmin[p]).  blockIdx.y is the simulation.
+__global__ __launch_bounds__(256) void k_fill_cell_cold(const SimDesc* __restrict__ sims) {
+    const SimDesc& S = sim_at(sims, blockIdx.y);
+    if (!S.lean) return;
+    const uint32_t nc = min(S.meta->ncells, S.cell_cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const CellT c = S.cells[i];
+        if ((c.info & kInfoLeaf) || c.start >= S.n) continue;
+        const uint32_t d = c.info & 255u;
+        const uint32_t ni = S.nbase[c.start] + (d - S.cdmin[c.start]);
+        if (ni < S.node_cap) S.ncold[ni] = NodeCold{c.start, c.end - c.start, c.T, d};
+    }
 }
 
 // ----------------------------------------------------------------------------

@@ -450,6 +450,13 @@ struct SimSet {
     void fill_body_cold(hipStream_t s) {
         hipLaunchKernelGGL(k_fill_body_cold, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
         HIP_OK(hipGetLastError());
+        // and those of a lean build's internal cells
+        bool lean = false;
+        for (const SimDesc& d : desc) lean = lean || d.lean != 0u;
+        if (lean) {
+            hipLaunchKernelGGL(k_fill_cell_cold, dim3(64, (uint32_t)desc.size()), dim3(256), 0, s, d_desc);
+            HIP_OK(hipGetLastError());
+        }
     }
     void final_kick(hipStream_t s) {
         hipLaunchKernelGGL(k_final_kick, dim3((uint32_t)t256.size()), dim3(kTile), 0, s, d_desc, d_t256);
