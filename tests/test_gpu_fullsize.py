@@ -7,7 +7,7 @@
   interaction-list hashes against the oracle's traversal of its final tree.
 - C2: the 1,048,576-body binary disc encounter: tree (keys, permutation, every
   cell record, sparse evidence and checksum) and forces with interaction
-  evidence, bit-exact.
+  evidence, bit-exact; and three drop-in steps against the oracle's trajectory.
 - C4: a server mix whose requests include 70,000 and 262,144 bodies among small
   ones; each result equals the oracle's trajectory and the same request run
   alone.
@@ -88,6 +88,33 @@ def test_c3_ensemble_256_at_bench_shape(oracle):
     assert not bad_ev, f"interaction evidence differs: {bad_ev[:10]}"
     inter = sum(int(e[0][:, 1].sum() + e[0][:, 2].sum()) for e in evid)
     print(f"  {inter} interactions per force state, all bit-exact", flush=True)
+
+
+def test_c2_full_size_trajectory(oracle, gpu, rt):
+    """C2 at 1,048,576 bodies through the drop-in loop (beyond-cap tree): the
+    initial force and three KDK steps, state and acceleration bit-exact against
+    the oracle's f32 trajectory."""
+    w = ics.config2()
+    s0 = w.states[0]
+    cfg, dt = w.config, w.dt
+    print(f"C2 trajectory: {len(s0)} bodies, 3 steps", flush=True)
+    ev = gpu.create_evolving_state(s0)
+    tree = rt.create_tree(gpu, ev, large=True)
+    rt.rebuild(gpu, ev, tree, cfg)
+    rt.force(gpu, ev, tree, cfg)
+    for _ in range(3):
+        gpu.kick_drift(ev, dt)
+        rt.rebuild(gpu, ev, tree, cfg)
+        rt.force(gpu, ev, tree, cfg)
+        gpu.final_kick(ev, dt)
+    s = gpu.read_evolving_state(ev)
+    a = gpu.read_evolving_accelerations(ev)
+    tree.close()
+    ev.close()
+    oracle.set_threads(THREADS)
+    rs, ra = oracle.f32_evolve(s0, 3, dt, ocfg(oracle, cfg))
+    assert same_bits(s, rs), "state differs from the oracle's trajectory"
+    assert same_bits(a, ra), "accelerations differ from the oracle's trajectory"
 
 
 def test_c2_full_size_tree_and_force(oracle, gpu, rt):
